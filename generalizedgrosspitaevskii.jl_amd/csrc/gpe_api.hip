@@ -1,0 +1,754 @@
+// libgpe_hip.so: C ABI (include/gpe_hip.h) over the fused gfx950 kernels.
+//
+// Host logic mirrors init/solve! of the reference:
+//   resolve_fixed_timestepping   src/fixed_time_stepping.jl:14-24
+//   solve! save/step loops       src/fixed_time_stepping.jl:26-54
+//   init (tables, pump buffers)  src/strang_splitting.jl:32-67
+//   step! ordering               src/strang_splitting.jl:86-90
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpe_hip.h"
+#include "gpe_launch.hpp"
+
+using namespace gpe;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                               \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(GPE_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+// ------------------------------------------------------------------ handle
+struct gpe_handle {
+  gpe_problem_desc d;
+  int M = 1, ndim = 1;
+  int64_t n0 = 0, n1 = 1, nsp = 0, batch = 1, npts = 0;
+  bool f64 = false;
+  size_t csz = 8;
+  hipStream_t stream = nullptr;
+  void* u[kMaxM] = {nullptr, nullptr};
+  void* tw = nullptr;
+  int64_t ntw = 0;
+  // dispersion
+  int disp_form = GPE_FORM_NONE;
+  double* dD = nullptr;
+  void* ed = nullptr;
+  double ed_dt = NAN;
+  // potential
+  int pot_form = GPE_FORM_NONE;
+  double* dV = nullptr;
+  void* ev[kMaxM] = {nullptr, nullptr};
+  double ev_dt = NAN;
+  // nonlinearity
+  bool has_g = false;
+  double g[kMaxM][kMaxM][2] = {};
+  double c[kMaxM][2] = {};
+  // pump
+  int pump_kind = GPE_PUMP_NONE;
+  void* pump_a[kMaxM] = {nullptr, nullptr};
+  void* pump_b[kMaxM] = {nullptr, nullptr};
+  std::vector<double> env;  // complex samples
+  double* d_env = nullptr;
+  int64_t d_env_cap = 0;
+  // noise
+  int noise_kind = GPE_NOISE_NONE;
+  double eta[kMaxM] = {0, 0};
+  uint64_t seed = 0, call = 0;
+  // staging for frames that go to host memory
+  void* stage[kMaxM] = {nullptr, nullptr};
+  // last solve configuration (for gpe_advance / timing)
+  bool configured = false;
+  double dt = 0;
+  float last_ms = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static size_t field_bytes(const gpe_handle* h) { return (size_t)h->npts * h->csz; }
+
+static bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+static hipError_t run_x(gpe_handle* h, const XPassArgs& a) {
+  const int64_t nlines = h->npts / h->n0;
+  return h->f64 ? gpe_run_x_f64(h->M, h->n0, a, nlines, h->stream) : gpe_run_x_f32(h->M, h->n0, a, nlines, h->stream);
+}
+static hipError_t run_y(gpe_handle* h, const YPassArgs& a) {
+  return h->f64 ? gpe_run_y_f64(h->M, h->n1, a, h->stream) : gpe_run_y_f32(h->M, h->n1, a, h->stream);
+}
+static hipError_t run_1d(gpe_handle* h, const Step1DArgs& a) {
+  return h->f64 ? gpe_run_1d_f64(h->M, h->n0, a, h->stream) : gpe_run_1d_f32(h->M, h->n0, a, h->stream);
+}
+
+// --------------------------------------------------------------- helpers
+static int upload(void** dst, const void* src, size_t bytes) {
+  if (*dst == nullptr) HIPCHK(hipMalloc(dst, bytes));
+  HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return GPE_OK;
+}
+
+static void dfree(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Twiddle table w[m] = exp(-2 pi i m / n), m = 0..n-1, in FP64: the angle is
+// reduced exactly to a quadrant and reflected to [0, pi/4] so every value is
+// one libm sin or cos of a small argument.
+static void twiddles_fp64(int64_t n, std::vector<double>& w) {
+  w.resize(2 * n);
+  for (int64_t m = 0; m < n; ++m) {
+    const int64_t q4 = 4 * m;
+    const int64_t quad = q4 / n;         // 0..3
+    const int64_t rem = q4 - quad * n;   // angle within quadrant = rem/n * pi/2
+    double c, s;
+    if (2 * rem <= n) {
+      const double phi = (double)rem / (double)n * (M_PI / 2);
+      c = std::cos(phi);
+      s = std::sin(phi);
+    } else {
+      const double phi = (double)(n - rem) / (double)n * (M_PI / 2);
+      c = std::sin(phi);
+      s = std::cos(phi);
+    }
+    double cq, sq;
+    switch (quad) {
+      case 0: cq = c; sq = s; break;
+      case 1: cq = -s; sq = c; break;
+      case 2: cq = -c; sq = -s; break;
+      default: cq = s; sq = -c; break;
+    }
+    w[2 * m] = cq;
+    w[2 * m + 1] = -sq;  // exp(-i angle)
+  }
+}
+
+// ------------------------------------------------------------------- ABI
+extern "C" int gpe_version(void) { return GPE_ABI_VERSION; }
+
+extern "C" const char* gpe_last_error(void) { return g_err.c_str(); }
+
+extern "C" int gpe_create(const gpe_problem_desc* desc, gpe_handle** out) {
+  if (!desc || !out) return fail(GPE_ERR_INVALID, "gpe_create: null argument");
+  *out = nullptr;
+  const gpe_problem_desc& d = *desc;
+  if (d.ndim < 1 || d.ndim > 2) return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ndim=%d (supported: 1, 2)", d.ndim);
+  if (d.ncomp < 1 || d.ncomp > kMaxM) return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ncomp=%d (supported: 1, 2)", d.ncomp);
+  if (d.dtype != GPE_C64 && d.dtype != GPE_C128) return fail(GPE_ERR_INVALID, "gpe_create: dtype=%d", d.dtype);
+  if (d.batch < 1) return fail(GPE_ERR_INVALID, "gpe_create: batch=%lld", (long long)d.batch);
+  for (int i = 0; i < d.ndim; ++i) {
+    if (!is_pow2(d.n[i]) || d.n[i] < 16 || d.n[i] > 4096)
+      return fail(GPE_ERR_UNSUPPORTED, "gpe_create: n[%d]=%lld (supported: powers of two 16..4096)", i, (long long)d.n[i]);
+    if (!(d.L[i] > 0) || !std::isfinite(d.L[i])) return fail(GPE_ERR_INVALID, "gpe_create: L[%d]=%g", i, d.L[i]);
+  }
+  if (d.dtype == GPE_C128 && d.ncomp == 2) {
+    for (int i = 0; i < d.ndim; ++i)
+      if (d.n[i] > 2048)
+        return fail(GPE_ERR_UNSUPPORTED, "gpe_create: n[%d]=%lld exceeds 2048 for two FP64 components (LDS)", i, (long long)d.n[i]);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GPE_ERR_DEVICE, "gpe_create: no HIP device");
+  if (d.device < 0 || d.device >= ndev) return fail(GPE_ERR_INVALID, "gpe_create: device=%d of %d", d.device, ndev);
+  HIPCHK(hipSetDevice(d.device));
+  gpe_handle* h = new gpe_handle();
+  h->d = d;
+  h->M = d.ncomp;
+  h->ndim = d.ndim;
+  h->n0 = d.n[0];
+  h->n1 = d.ndim == 2 ? d.n[1] : 1;
+  h->nsp = h->n0 * h->n1;
+  h->batch = d.batch;
+  h->npts = h->nsp * h->batch;
+  h->f64 = d.dtype == GPE_C128;
+  h->csz = h->f64 ? 16 : 8;
+  int rc = GPE_OK;
+  auto cleanup = [&](int code) {
+    gpe_destroy(h);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(GPE_ERR_DEVICE, "gpe_create: stream"));
+  for (int c = 0; c < h->M; ++c)
+    if (hipMalloc(&h->u[c], field_bytes(h)) != hipSuccess)
+      return cleanup(fail(GPE_ERR_NOMEM, "gpe_create: field allocation of %zu bytes", field_bytes(h)));
+  h->ntw = std::max(h->n0, h->n1);
+  std::vector<double> w;
+  twiddles_fp64(h->ntw, w);
+  if (h->f64) {
+    rc = upload(&h->tw, w.data(), w.size() * sizeof(double));
+  } else {
+    std::vector<float> wf(w.begin(), w.end());
+    rc = upload(&h->tw, wf.data(), wf.size() * sizeof(float));
+  }
+  if (rc) return cleanup(rc);
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return cleanup(fail(GPE_ERR_DEVICE, "gpe_create: events"));
+  *out = h;
+  return GPE_OK;
+}
+
+extern "C" int gpe_destroy(gpe_handle* h) {
+  if (!h) return GPE_OK;
+  (void)hipSetDevice(h->d.device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int c = 0; c < kMaxM; ++c) {
+    dfree(h->u[c]);
+    dfree(h->ev[c]);
+    dfree(h->pump_a[c]);
+    dfree(h->pump_b[c]);
+    dfree(h->stage[c]);
+  }
+  dfree(h->tw);
+  void* p = h->dD;
+  dfree(p);
+  p = h->dV;
+  dfree(p);
+  dfree(h->ed);
+  p = h->d_env;
+  dfree(p);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return GPE_OK;
+}
+
+static int per_point(int form, int M) { return form == GPE_FORM_SCALAR ? 1 : (form == GPE_FORM_DIAG ? M : M * M); }
+
+extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: null handle");
+  HIPCHK(hipSetDevice(h->d.device));
+  if (form == GPE_FORM_NONE) {
+    h->disp_form = GPE_FORM_NONE;
+    h->ed_dt = NAN;
+    return GPE_OK;
+  }
+  if (form < GPE_FORM_SCALAR || form > GPE_FORM_FULL) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: form=%d", form);
+  if (!D) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: null table");
+  const size_t bytes = (size_t)h->nsp * per_point(form, h->M) * 2 * sizeof(double);
+  void* p = h->dD;
+  dfree(p);
+  h->dD = nullptr;
+  dfree(h->ed);
+  int rc = upload(&p, D, bytes);
+  h->dD = (double*)p;
+  if (rc) return rc;
+  HIPCHK(hipMalloc(&h->ed, (size_t)h->nsp * per_point(form, h->M) * h->csz));
+  h->disp_form = form;
+  h->ed_dt = NAN;
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_potential(gpe_handle* h, int32_t form, const double* V) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_potential: null handle");
+  HIPCHK(hipSetDevice(h->d.device));
+  void* p = h->dV;
+  dfree(p);
+  h->dV = nullptr;
+  for (int c = 0; c < kMaxM; ++c) dfree(h->ev[c]);
+  h->ev_dt = NAN;
+  if (form == GPE_FORM_NONE) {
+    h->pot_form = GPE_FORM_NONE;
+    return GPE_OK;
+  }
+  if (form != GPE_FORM_SCALAR && form != GPE_FORM_DIAG)
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_set_potential: form=%d (supported: scalar, diagonal)", form);
+  if (!V) return fail(GPE_ERR_INVALID, "gpe_set_potential: null table");
+  int rc = upload(&p, V, (size_t)h->nsp * per_point(form, h->M) * 2 * sizeof(double));
+  h->dV = (double*)p;
+  if (rc) return rc;
+  for (int c = 0; c < h->M; ++c) HIPCHK(hipMalloc(&h->ev[c], (size_t)h->nsp * h->csz));
+  h->pot_form = form;
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double* c) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_nonlinearity: null handle");
+  const int M = h->M;
+  h->has_g = (g != nullptr) || (c != nullptr);
+  for (int i = 0; i < M; ++i) {
+    for (int j = 0; j < M; ++j)
+      for (int r = 0; r < 2; ++r) h->g[i][j][r] = g ? g[2 * (i + M * j) + r] : 0.0;
+    for (int r = 0; r < 2; ++r) h->c[i][r] = c ? c[2 * i + r] : 0.0;
+  }
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, const double* envelope, int64_t n_env) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_pump: null handle");
+  HIPCHK(hipSetDevice(h->d.device));
+  for (int c = 0; c < kMaxM; ++c) {
+    dfree(h->pump_a[c]);
+    dfree(h->pump_b[c]);
+  }
+  h->env.clear();
+  h->pump_kind = GPE_PUMP_NONE;
+  if (kind == GPE_PUMP_NONE) return GPE_OK;
+  if (kind != GPE_PUMP_TABLE && kind != GPE_PUMP_SEPARABLE) return fail(GPE_ERR_INVALID, "gpe_set_pump: kind=%d", kind);
+  if (!profile) return fail(GPE_ERR_INVALID, "gpe_set_pump: null profile");
+  if (n_env < 0 || (n_env > 0 && !envelope)) return fail(GPE_ERR_INVALID, "gpe_set_pump: envelope");
+  const int64_t per = kind == GPE_PUMP_TABLE ? h->nsp : (h->n0 + (h->ndim == 2 ? h->n1 : 0));
+  for (int c = 0; c < h->M; ++c) {
+    const double* pc = profile + 2 * per * c;
+    bool nonzero = false;
+    for (int64_t i = 0; i < 2 * per && !nonzero; ++i) nonzero = pc[i] != 0.0;
+    if (!nonzero) continue;  // component not pumped: F_c = 0 exactly
+    std::vector<double> tmp;
+    auto put = [&](void** dst, const double* src, int64_t n) -> int {
+      if (h->f64) return upload(dst, src, (size_t)n * 16);
+      std::vector<float> f(2 * n);
+      for (int64_t i = 0; i < 2 * n; ++i) f[i] = (float)src[i];
+      return upload(dst, f.data(), (size_t)n * 8);
+    };
+    int rc;
+    if (kind == GPE_PUMP_TABLE) {
+      rc = put(&h->pump_a[c], pc, h->nsp);
+    } else {
+      rc = put(&h->pump_a[c], pc, h->n0);
+      if (!rc && h->ndim == 2) rc = put(&h->pump_b[c], pc + 2 * h->n0, h->n1);
+    }
+    if (rc) return rc;
+  }
+  h->env.assign(envelope, envelope + 2 * n_env);
+  h->pump_kind = kind;
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_noise(gpe_handle* h, int32_t kind, const double* eta, uint64_t seed, uint64_t call0) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_noise: null handle");
+  if (kind != GPE_NOISE_NONE && kind != GPE_NOISE_COMPLEX && kind != GPE_NOISE_REAL)
+    return fail(GPE_ERR_INVALID, "gpe_set_noise: kind=%d", kind);
+  if (kind != GPE_NOISE_NONE && !eta) return fail(GPE_ERR_INVALID, "gpe_set_noise: null eta");
+  h->noise_kind = kind;
+  for (int c = 0; c < kMaxM; ++c) h->eta[c] = (kind != GPE_NOISE_NONE && c < h->M) ? eta[c] : 0.0;
+  h->seed = seed;
+  h->call = call0;
+  return GPE_OK;
+}
+
+extern "C" int gpe_resolve_timestepping(double dt, double t0, double t1, int64_t nsaves, double* dt_out,
+                                        int64_t* steps_per_save) {
+  if (!(dt > 0) || nsaves < 1 || !dt_out || !steps_per_save || !(t1 > t0))
+    return fail(GPE_ERR_INVALID, "gpe_resolve_timestepping: dt=%g t0=%g t1=%g nsaves=%lld", dt, t0, t1, (long long)nsaves);
+  const double dT = (t1 - t0) / (double)nsaves;
+  const double q = std::ceil(dT / dt);
+  if (!(q >= 1) || q > 9.0e15) return fail(GPE_ERR_INVALID, "gpe_resolve_timestepping: steps_per_save=%g", q);
+  *steps_per_save = (int64_t)q;
+  *dt_out = dT / (double)(int64_t)q;
+  return GPE_OK;
+}
+
+extern "C" int gpe_pump_times(double t0, double t1, double dt, int64_t nsaves, double* times, int64_t n) {
+  double rdt;
+  int64_t sps;
+  int rc = gpe_resolve_timestepping(dt, t0, t1, nsaves, &rdt, &sps);
+  if (rc) return rc;
+  const int64_t S = sps * nsaves;
+  if (!times || n != 1 + 2 * S) return fail(GPE_ERR_INVALID, "gpe_pump_times: n=%lld, expected %lld", (long long)n, (long long)(1 + 2 * S));
+  double t = t0;
+  times[0] = t0;
+  for (int64_t s = 1; s <= S; ++s) {
+    t += rdt;
+    times[2 * s - 1] = t + rdt / 2;
+    times[2 * s] = t + rdt;
+  }
+  return GPE_OK;
+}
+
+// Build exp(-i dt D)/(prod n) and exp(-i dt/2 V) for the resolved dt.
+static int prepare_tables(gpe_handle* h, double dt) {
+  if (h->disp_form != GPE_FORM_NONE && !(h->ed_dt == dt)) {
+    EdArgs a;
+    a.D = h->dD;
+    a.out = h->ed;
+    a.npts = h->nsp;
+    a.form = h->disp_form;
+    a.M = h->M;
+    a.dt = dt;
+    a.scale = 1.0 / (double)h->nsp;
+    const unsigned blocks = (unsigned)((h->nsp + 255) / 256);
+    if (h->f64)
+      hipLaunchKernelGGL(ed_table_kernel<double>, dim3(blocks), dim3(256), 0, h->stream, a);
+    else
+      hipLaunchKernelGGL(ed_table_kernel<float>, dim3(blocks), dim3(256), 0, h->stream, a);
+    HIPCHK(hipGetLastError());
+    h->ed_dt = dt;
+  }
+  if (h->pot_form != GPE_FORM_NONE && !(h->ev_dt == dt)) {
+    EvArgs a;
+    a.V = h->dV;
+    for (int c = 0; c < kMaxM; ++c) a.out[c] = h->ev[c];
+    a.npts = h->nsp;
+    a.form = h->pot_form;
+    a.M = h->M;
+    a.tau = dt / 2;
+    const unsigned blocks = (unsigned)((h->nsp + 255) / 256);
+    if (h->f64)
+      hipLaunchKernelGGL(ev_table_kernel<double>, dim3(blocks), dim3(256), 0, h->stream, a);
+    else
+      hipLaunchKernelGGL(ev_table_kernel<float>, dim3(blocks), dim3(256), 0, h->stream, a);
+    HIPCHK(hipGetLastError());
+    h->ev_dt = dt;
+  }
+  return GPE_OK;
+}
+
+static PointTerms point_terms(const gpe_handle* h) {
+  PointTerms pt;
+  memset(&pt, 0, sizeof pt);
+  pt.has_g = h->has_g;
+  memcpy(pt.g, h->g, sizeof pt.g);
+  memcpy(pt.c, h->c, sizeof pt.c);
+  pt.pump_kind = h->pump_kind;
+  for (int c = 0; c < kMaxM; ++c) {
+    pt.pump_a[c] = h->pump_a[c];
+    pt.pump_b[c] = h->pump_b[c];
+    pt.pot[c] = h->ev[c];
+  }
+  pt.pot_kind = h->pot_form != GPE_FORM_NONE;
+  pt.noise_kind = h->noise_kind;
+  for (int c = 0; c < kMaxM; ++c) pt.eta[c] = h->eta[c];
+  pt.seed_lo = (uint32_t)h->seed;
+  pt.seed_hi = (uint32_t)(h->seed >> 32);
+  return pt;
+}
+
+// envelope sample i (complex); 1 when the pump is static
+static void env_at(const gpe_handle* h, int64_t i, double out[2]) {
+  if (h->env.empty()) {
+    out[0] = 1.0;
+    out[1] = 0.0;
+  } else {
+    out[0] = h->env[2 * i];
+    out[1] = h->env[2 * i + 1];
+  }
+}
+
+// Half-step `half` (0: at t_s + dt/2, 1: at t_s + dt) of step s (1-based).
+static HalfStep make_half(const gpe_handle* h, double dt, int64_t s, int half, uint64_t call0) {
+  HalfStep hs;
+  hs.delta = dt / 2;
+  const double q = hs.delta / 2;
+  double en[2], ex[2];
+  env_at(h, 2 * s - 2 + half, en);
+  env_at(h, 2 * s - 1 + half, ex);
+  hs.fnow[0] = q * en[0];
+  hs.fnow[1] = q * en[1];
+  hs.fnext[0] = q * ex[0];
+  hs.fnext[1] = q * ex[1];
+  hs.call = call0 + 2 * (uint64_t)(s - 1) + (uint64_t)half;
+  return hs;
+}
+
+static XPassArgs x_args(const gpe_handle* h) {
+  XPassArgs a;
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < kMaxM; ++c) a.u[c] = h->u[c];
+  a.nlines = h->npts / h->n0;
+  a.ny = h->n1;
+  a.tw = h->tw;
+  a.tws = (int)(h->ntw / h->n0);
+  a.pt = point_terms(h);
+  return a;
+}
+
+static YPassArgs y_args(const gpe_handle* h) {
+  YPassArgs a;
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < kMaxM; ++c) a.u[c] = h->u[c];
+  const int C = y_cols_per_wg((int)h->n1);
+  a.nx = h->n0;
+  a.ncolblk = h->n0 / C;
+  a.nblocks = a.ncolblk * h->batch;
+  a.xcd_remap = (a.nblocks % 8 == 0) ? 1 : 0;
+  a.ed_form = h->disp_form == GPE_FORM_NONE ? 0 : h->disp_form;
+  a.ed = h->ed;
+  a.scale = 1.0 / (double)h->nsp;
+  a.tw = h->tw;
+  a.tws = (int)(h->ntw / h->n1);
+  return a;
+}
+
+static Step1DArgs s1_args(gpe_handle* h, double dt) {
+  Step1DArgs a;
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < kMaxM; ++c) a.u[c] = h->u[c];
+  a.nlines = h->batch;
+  a.delta = dt / 2;
+  a.ed_form = h->disp_form == GPE_FORM_NONE ? 0 : h->disp_form;
+  a.ed = h->ed;
+  a.scale = 1.0 / (double)h->nsp;
+  a.tw = h->tw;
+  a.tws = (int)(h->ntw / h->n0);
+  a.pt = point_terms(h);
+  a.env = h->env.empty() ? nullptr : h->d_env;
+  return a;
+}
+
+static int ensure_stage(gpe_handle* h) {
+  for (int c = 0; c < h->M; ++c)
+    if (!h->stage[c]) HIPCHK(hipMalloc(&h->stage[c], field_bytes(h)));
+  return GPE_OK;
+}
+
+// Frame destination pointers for frame index f of result[], staging when the
+// caller's memory is on the host.
+struct FrameSink {
+  bool to_host = false;
+  void* kernel_dst[kMaxM] = {nullptr, nullptr};
+  char* host_dst[kMaxM] = {nullptr, nullptr};
+};
+
+extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double dt, int64_t nsaves,
+                         int32_t save_start, void* const* result, double* ts) {
+  if (!h || !u0 || !result) return fail(GPE_ERR_INVALID, "gpe_solve: null argument");
+  for (int c = 0; c < h->M; ++c)
+    if (!u0[c] || !result[c]) return fail(GPE_ERR_INVALID, "gpe_solve: null field pointer for component %d", c);
+  HIPCHK(hipSetDevice(h->d.device));
+  double rdt;
+  int64_t sps;
+  int rc = gpe_resolve_timestepping(dt, t0, t1, nsaves, &rdt, &sps);
+  if (rc) return rc;
+  const int64_t S = sps * nsaves;
+  if (h->pump_kind != GPE_PUMP_NONE && !h->env.empty() && (int64_t)h->env.size() != 2 * (1 + 2 * S))
+    return fail(GPE_ERR_STATE, "gpe_solve: pump envelope has %lld samples, this solve needs %lld (gpe_pump_times)",
+                (long long)(h->env.size() / 2), (long long)(1 + 2 * S));
+  if (h->ndim == 1 && !h->env.empty()) {
+    const int64_t need = (int64_t)h->env.size();
+    if (need > h->d_env_cap) {
+      void* p = h->d_env;
+      dfree(p);
+      h->d_env = nullptr;
+      HIPCHK(hipMalloc(&p, need * sizeof(double)));
+      h->d_env = (double*)p;
+      h->d_env_cap = need;
+    }
+    HIPCHK(hipMemcpyAsync(h->d_env, h->env.data(), need * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  }
+  rc = prepare_tables(h, rdt);
+  if (rc) return rc;
+  const size_t fb = field_bytes(h);
+  for (int c = 0; c < h->M; ++c) HIPCHK(hipMemcpyAsync(h->u[c], u0[c], fb, hipMemcpyDefault, h->stream));
+  const int64_t nfr = nsaves + (save_start ? 1 : 0);
+  bool res_dev = true;
+  for (int c = 0; c < h->M; ++c) res_dev = res_dev && is_device_ptr(result[c]);
+  if (!res_dev) {
+    rc = ensure_stage(h);
+    if (rc) return rc;
+  }
+  if (save_start)
+    for (int c = 0; c < h->M; ++c) HIPCHK(hipMemcpyAsync(result[c], u0[c], fb, hipMemcpyDefault, h->stream));
+  auto frame_ptr = [&](int c, int64_t f) -> void* {
+    return res_dev ? (void*)((char*)result[c] + (size_t)f * fb) : h->stage[c];
+  };
+  auto flush_frame = [&](int64_t f) -> int {
+    if (res_dev) return GPE_OK;
+    for (int c = 0; c < h->M; ++c)
+      HIPCHK(hipMemcpyAsync((char*)result[c] + (size_t)f * fb, h->stage[c], fb, hipMemcpyDeviceToHost, h->stream));
+    return GPE_OK;
+  };
+  const uint64_t call0 = h->call;
+  double t = t0;
+  if (ts) ts[0] = t0;
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  if (h->ndim == 2) {
+    XPassArgs xa = x_args(h);
+    const YPassArgs ya = y_args(h);
+    xa.flags = kStart | kOutK;
+    xa.h_start = make_half(h, rdt, 1, 0, call0);
+    HIPCHK(run_x(h, xa));
+    for (int64_t s = 1; s <= S; ++s) {
+      t += rdt;
+      HIPCHK(run_y(h, ya));
+      const bool save = (s % sps) == 0;
+      const int64_t f = s / sps - 1 + (save_start ? 1 : 0);
+      xa.h_end = make_half(h, rdt, s, 1, call0);
+      for (int c = 0; c < kMaxM; ++c) xa.frame[c] = (save && c < h->M) ? frame_ptr(c, f) : nullptr;
+      if (s < S) {
+        xa.flags = kInK | kEnd | kStart | kOutK;
+        xa.h_start = make_half(h, rdt, s + 1, 0, call0);
+      } else {
+        xa.flags = kInK | kEnd;
+      }
+      HIPCHK(run_x(h, xa));
+      if (save) {
+        rc = flush_frame(f);
+        if (rc) return rc;
+        if (ts) ts[s / sps - (save_start ? 0 : 1)] = t;
+      }
+    }
+  } else {
+    Step1DArgs a = s1_args(h, rdt);
+    for (int64_t n = 0; n < nsaves; ++n) {
+      a.nsteps = sps;
+      a.env_base = 2 * n * sps;
+      a.call0 = call0 + 2 * (uint64_t)(n * sps);
+      const int64_t f = n + (save_start ? 1 : 0);
+      for (int c = 0; c < kMaxM; ++c) a.frame[c] = c < h->M ? frame_ptr(c, f) : nullptr;
+      HIPCHK(run_1d(h, a));
+      rc = flush_frame(f);
+      if (rc) return rc;
+      for (int64_t k = 0; k < sps; ++k) t += rdt;
+      if (ts) ts[n + (save_start ? 1 : 0)] = t;
+    }
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  h->call = call0 + 2 * (uint64_t)S;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  h->configured = true;
+  h->dt = rdt;
+  (void)nfr;
+  return GPE_OK;
+}
+
+extern "C" int gpe_get_field(gpe_handle* h, void* const* u) {
+  if (!h || !u) return fail(GPE_ERR_INVALID, "gpe_get_field: null argument");
+  HIPCHK(hipSetDevice(h->d.device));
+  for (int c = 0; c < h->M; ++c) HIPCHK(hipMemcpyAsync(u[c], h->u[c], field_bytes(h), hipMemcpyDefault, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
+extern "C" int gpe_last_solve_ms(gpe_handle* h, float* ms) {
+  if (!h || !ms) return fail(GPE_ERR_INVALID, "gpe_last_solve_ms: null argument");
+  *ms = h->last_ms;
+  return GPE_OK;
+}
+
+// Continue from the current field with the last solve's dt and the envelope
+// frozen at its last sample, no frame output.  With per_kernel, HIP events
+// bracket every launch and the mean durations are returned.
+static int advance_impl(gpe_handle* h, int64_t nsteps, bool per_kernel, double* ms_x, double* ms_y) {
+  if (!h->configured) return fail(GPE_ERR_STATE, "gpe_advance: no previous gpe_solve");
+  if (nsteps < 1) return fail(GPE_ERR_INVALID, "gpe_advance: nsteps=%lld", (long long)nsteps);
+  HIPCHK(hipSetDevice(h->d.device));
+  const std::vector<double> saved_env = h->env;
+  if (!h->env.empty()) {
+    const double re = h->env[h->env.size() - 2], im = h->env[h->env.size() - 1];
+    h->env.assign(2 * (1 + 2 * nsteps), 0.0);
+    for (int64_t i = 0; i < 1 + 2 * nsteps; ++i) {
+      h->env[2 * i] = re;
+      h->env[2 * i + 1] = im;
+    }
+    if (h->ndim == 1) {
+      const int64_t need = (int64_t)h->env.size();
+      if (need > h->d_env_cap) {
+        void* p = h->d_env;
+        dfree(p);
+        h->d_env = nullptr;
+        HIPCHK(hipMalloc(&p, need * sizeof(double)));
+        h->d_env = (double*)p;
+        h->d_env_cap = need;
+      }
+      HIPCHK(hipMemcpyAsync(h->d_env, h->env.data(), need * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+  }
+  std::vector<hipEvent_t> evs;
+  std::vector<int> kinds;
+  auto launch = [&](int kind, auto&& fn) -> int {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (per_kernel) {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipEventRecord(e0, h->stream));
+    }
+    HIPCHK(fn());
+    if (per_kernel) {
+      HIPCHK(hipEventRecord(e1, h->stream));
+      evs.push_back(e0);
+      evs.push_back(e1);
+      kinds.push_back(kind);
+    }
+    return GPE_OK;
+  };
+  int rc = GPE_OK;
+  const uint64_t call0 = h->call;
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  if (h->ndim == 2) {
+    XPassArgs xa = x_args(h);
+    const YPassArgs ya = y_args(h);
+    xa.flags = kStart | kOutK;
+    xa.h_start = make_half(h, h->dt, 1, 0, call0);
+    rc = launch(0, [&] { return run_x(h, xa); });
+    for (int64_t s = 1; s <= nsteps && rc == GPE_OK; ++s) {
+      rc = launch(1, [&] { return run_y(h, ya); });
+      if (rc) break;
+      xa.h_end = make_half(h, h->dt, s, 1, call0);
+      if (s < nsteps) {
+        xa.flags = kInK | kEnd | kStart | kOutK;
+        xa.h_start = make_half(h, h->dt, s + 1, 0, call0);
+      } else {
+        xa.flags = kInK | kEnd;
+      }
+      rc = launch(0, [&] { return run_x(h, xa); });
+    }
+  } else {
+    Step1DArgs a = s1_args(h, h->dt);
+    a.nsteps = nsteps;
+    a.env_base = 0;
+    a.call0 = call0;
+    rc = launch(2, [&] { return run_1d(h, a); });
+  }
+  h->env = saved_env;
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  h->call = call0 + 2 * (uint64_t)nsteps;
+  if (per_kernel) {
+    double acc[3] = {0, 0, 0};
+    int64_t cnt[3] = {0, 0, 0};
+    for (size_t i = 0; i < kinds.size(); ++i) {
+      float ms;
+      HIPCHK(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+      acc[kinds[i]] += ms;
+      cnt[kinds[i]]++;
+    }
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    if (ms_x) *ms_x = h->ndim == 2 ? (cnt[0] ? acc[0] / cnt[0] : 0) : (cnt[2] ? acc[2] / cnt[2] : 0);
+    if (ms_y) *ms_y = cnt[1] ? acc[1] / cnt[1] : 0;
+  }
+  return GPE_OK;
+}
+
+extern "C" int gpe_advance(gpe_handle* h, int64_t nsteps, float* ms) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_advance: null handle");
+  int rc = advance_impl(h, nsteps, false, nullptr, nullptr);
+  if (rc) return rc;
+  if (ms) *ms = h->last_ms;
+  return GPE_OK;
+}
+
+extern "C" int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_profile_kernels: null handle");
+  return advance_impl(h, nsteps, true, ms_x, ms_y);
+}

@@ -1,0 +1,71 @@
+// Template launchers shared by the per-precision instantiation units.
+#pragma once
+#include "gpe_launch.hpp"
+
+namespace gpe {
+
+template <typename K>
+static hipError_t set_lds(K k, size_t lds) {
+  if (lds > 65536) return hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return hipSuccess;
+}
+
+template <typename R, int N, int M>
+static hipError_t launch_x(const XPassArgs& a, int64_t nlines, hipStream_t s) {
+  constexpr int W = x_lines_per_wg(N);
+  constexpr int T = N / kE;
+  const size_t lds = 2ull * M * W * lds_line_stride(N) * sizeof(Cx<R>);
+  auto k = x_pass_kernel<R, N, M, W>;
+  hipError_t e = set_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)((nlines + W - 1) / W)), dim3(W * T), lds, s, a);
+  return hipGetLastError();
+}
+
+template <typename R, int N, int M>
+static hipError_t launch_y(const YPassArgs& a, hipStream_t s) {
+  constexpr int T = N / kE;
+  constexpr int C = y_cols_per_wg(N);
+  const size_t lds = 2ull * M * C * lds_line_stride(N) * sizeof(Cx<R>);
+  auto k = y_pass_kernel<R, N, M, C>;
+  hipError_t e = set_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)a.nblocks), dim3(C * T), lds, s, a);
+  return hipGetLastError();
+}
+
+template <typename R, int N, int M>
+static hipError_t launch_1d(const Step1DArgs& a, hipStream_t s) {
+  constexpr int W = x_lines_per_wg(N);
+  constexpr int T = N / kE;
+  const size_t lds = 2ull * M * W * lds_line_stride(N) * sizeof(Cx<R>);
+  auto k = step1d_kernel<R, N, M, W>;
+  hipError_t e = set_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)((a.nlines + W - 1) / W)), dim3(W * T), lds, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace gpe
+
+#define GPE_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
+#define GPE_DEFINE_RUNNERS(R, SUFFIX)                                                                  \
+  hipError_t gpe_run_x_##SUFFIX(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, hipStream_t s) { \
+    switch (N * 4 + M) {                                                                               \
+      GPE_SIZES(GPE_XCASE_##SUFFIX)                                                                    \
+    }                                                                                                  \
+    return hipErrorInvalidValue;                                                                       \
+  }                                                                                                    \
+  hipError_t gpe_run_y_##SUFFIX(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s) {           \
+    switch (N * 4 + M) {                                                                               \
+      GPE_SIZES(GPE_YCASE_##SUFFIX)                                                                    \
+    }                                                                                                  \
+    return hipErrorInvalidValue;                                                                       \
+  }                                                                                                    \
+  hipError_t gpe_run_1d_##SUFFIX(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s) {         \
+    switch (N * 4 + M) {                                                                               \
+      GPE_SIZES(GPE_1CASE_##SUFFIX)                                                                    \
+    }                                                                                                  \
+    return hipErrorInvalidValue;                                                                       \
+  }

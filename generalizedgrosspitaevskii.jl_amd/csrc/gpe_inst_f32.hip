@@ -1,0 +1,14 @@
+// FP32 (ComplexF32 field) instantiations of the fused kernels.
+#include "gpe_inst.hpp"
+
+#define GPE_XCASE_f32(n)                                       \
+  case n * 4 + 1: return gpe::launch_x<float, n, 1>(a, nlines, s); \
+  case n * 4 + 2: return gpe::launch_x<float, n, 2>(a, nlines, s);
+#define GPE_YCASE_f32(n)                                  \
+  case n * 4 + 1: return gpe::launch_y<float, n, 1>(a, s); \
+  case n * 4 + 2: return gpe::launch_y<float, n, 2>(a, s);
+#define GPE_1CASE_f32(n)                                   \
+  case n * 4 + 1: return gpe::launch_1d<float, n, 1>(a, s); \
+  case n * 4 + 2: return gpe::launch_1d<float, n, 2>(a, s);
+
+GPE_DEFINE_RUNNERS(float, f32)

@@ -1,0 +1,15 @@
+// FP64 (ComplexF64 field) instantiations of the fused kernels.  Two FP64
+// components at N = 4096 do not fit the LDS and are rejected by gpe_create.
+#include "gpe_inst.hpp"
+
+#define GPE_XCASE_f64(n)                                        \
+  case n * 4 + 1: return gpe::launch_x<double, n, 1>(a, nlines, s); \
+  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_x<double, (n > 2048 ? 2048 : n), 2>(a, nlines, s);
+#define GPE_YCASE_f64(n)                                   \
+  case n * 4 + 1: return gpe::launch_y<double, n, 1>(a, s); \
+  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_y<double, (n > 2048 ? 2048 : n), 2>(a, s);
+#define GPE_1CASE_f64(n)                                    \
+  case n * 4 + 1: return gpe::launch_1d<double, n, 1>(a, s); \
+  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_1d<double, (n > 2048 ? 2048 : n), 2>(a, s);
+
+GPE_DEFINE_RUNNERS(double, f64)

@@ -1,0 +1,23 @@
+// Launch shapes and the per-precision launch entry points of the fused
+// kernels (instantiated in gpe_inst_f32.hip / gpe_inst_f64.hip).
+#pragma once
+#include "gpe_kernels.hpp"
+
+namespace gpe {
+
+// Lines per workgroup of the x-pass and of the 1D whole-step kernel.
+__host__ __device__ constexpr int x_lines_per_wg(int N) { return N >= 1024 ? 1 : 1024 / N; }
+
+// Columns per workgroup of the y-pass: about 256 threads, at most 16 columns.
+__host__ __device__ constexpr int y_cols_per_wg(int N) {
+  return (256 / (N / kE)) < 1 ? 1 : ((256 / (N / kE)) > 16 ? 16 : 256 / (N / kE));
+}
+
+}  // namespace gpe
+
+hipError_t gpe_run_x_f32(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, hipStream_t s);
+hipError_t gpe_run_x_f64(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, hipStream_t s);
+hipError_t gpe_run_y_f32(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s);
+hipError_t gpe_run_y_f64(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s);
+hipError_t gpe_run_1d_f32(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);
+hipError_t gpe_run_1d_f64(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);

@@ -1,0 +1,153 @@
+/*
+ * gpe_hip.h -- C ABI of libgpe_hip.so, the MI355X (gfx950) implementation of
+ * the Strang-splitting time step behind
+ *     solve(prob, StrangSplitting(), tspan; dt, nsaves)
+ * of GeneralizedGrossPitaevskii.jl.
+ *
+ * Conventions (identical to Julia's memory layout, so a Julia caller passes
+ * its arrays unchanged):
+ *   - every field is a column-major array, dimension 0 contiguous, complex
+ *     values interleaved (re, im);
+ *   - the M components are separate arrays (the reference's NTuple{M} u0);
+ *   - dimensions beyond the `ndim` spatial ones are batch (trajectory)
+ *     dimensions, flattened into `batch`;
+ *   - host-side tables (dispersion, potential, pump) are complex FP64;
+ *   - pointers to fields and results may be host or device memory.
+ * Every function returns GPE_OK (0) or a negative GPE_ERR_* code; the text of
+ * the last error of the calling thread is returned by gpe_last_error().
+ * A handle is not re-entrant; distinct handles may be used concurrently.
+ */
+#ifndef GPE_HIP_H
+#define GPE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPE_ABI_VERSION 1
+
+enum gpe_status {
+  GPE_OK = 0,
+  GPE_ERR_INVALID = -1,     /* bad argument (shape, size, pointer, value)      */
+  GPE_ERR_UNSUPPORTED = -2, /* valid in the reference, not supported here      */
+  GPE_ERR_DEVICE = -3,      /* HIP runtime error                               */
+  GPE_ERR_STATE = -4,       /* call out of order (e.g. envelope length)        */
+  GPE_ERR_NOMEM = -5        /* device allocation failed                        */
+};
+
+enum gpe_dtype { GPE_C64 = 0 /* ComplexF32 fields */, GPE_C128 = 1 /* ComplexF64 fields */ };
+
+/* Static-array form of a per-point term, as returned by the reference's user
+ * closures: a scalar, an SVector{M} (elementwise) or an SMatrix{M,M}. */
+enum gpe_form { GPE_FORM_NONE = 0, GPE_FORM_SCALAR = 1, GPE_FORM_DIAG = 2, GPE_FORM_FULL = 3 };
+
+enum gpe_pump_kind { GPE_PUMP_NONE = 0, GPE_PUMP_TABLE = 1, GPE_PUMP_SEPARABLE = 2 };
+
+enum gpe_noise_kind { GPE_NOISE_NONE = 0, GPE_NOISE_COMPLEX = 1, GPE_NOISE_REAL = 2 };
+
+typedef struct gpe_problem_desc {
+  int32_t ndim;   /* spatial dimensions: 1 or 2 (length(lengths))             */
+  int32_t ncomp;  /* M components: 1 or 2 (length(u0))                        */
+  int64_t n[3];   /* grid points per spatial dimension (powers of two, 16..4096) */
+  double L[3];    /* box lengths (prob.lengths)                               */
+  int64_t batch;  /* product of the trailing (trajectory) dimensions, >= 1    */
+  int32_t dtype;  /* enum gpe_dtype                                           */
+  int32_t device; /* HIP device ordinal                                       */
+} gpe_problem_desc;
+
+typedef struct gpe_handle gpe_handle;
+
+/* Version of this ABI (GPE_ABI_VERSION). */
+int gpe_version(void);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* gpe_last_error(void);
+
+/* Allocate device state for one problem shape.  Replaces the allocation half
+ * of init(prob, ::StrangSplitting, tspan) (src/strang_splitting.jl:41-49) and
+ * the checks of GrossPitaevskiiProblem (src/problem.jl:112-115). */
+int gpe_create(const gpe_problem_desc* desc, gpe_handle** out);
+
+/* Release everything owned by the handle. */
+int gpe_destroy(gpe_handle* h);
+
+/* Dispersion D(k) sampled on the reciprocal grid (src/problem.jl:135-139) in
+ * FFT order, column-major over the spatial dims, complex FP64:
+ *   GPE_FORM_SCALAR: 1 value per k point
+ *   GPE_FORM_DIAG:   M values per k point (component fastest)
+ *   GPE_FORM_FULL:   M*M values per k point (column-major matrix)
+ *   GPE_FORM_NONE:   no dispersion (D = AdditiveIdentity), D must be NULL.
+ * gpe_solve turns it into exp(-i dt D(k)) on the device (per-k matrix
+ * exponential for FULL).  Replaces get_exponential(dispersion, ...)
+ * (src/misc.jl:12-20, src/strang_splitting.jl:53). */
+int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D);
+
+/* Potential V(r) on the direct grid (src/problem.jl:129-133), SCALAR or DIAG
+ * form as above; exp(-i dt/2 V) is formed on the device.  Replaces
+ * get_exponential(potential, ...) (src/strang_splitting.jl:54). */
+int gpe_set_potential(gpe_handle* h, int32_t form, const double* V);
+
+/* Nonlinearity G(u) of the descriptor family
+ *     G_i(u) = c[i] + sum_j g[i + M*j] |u_j|^2        (complex FP64)
+ * which the real-space half-step applies as cis(-dt/2 G(u))
+ * (src/kernels.jl:44).  g == NULL and c == NULL clears it. */
+int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double* c);
+
+/* Pump F(r,t) = profile(r) * envelope(t) (src/misc.jl:22-42).
+ *   GPE_PUMP_TABLE:     profile = M blocks of prod(n) complex values.
+ *   GPE_PUMP_SEPARABLE: profile = M blocks of (n[0] + ... + n[ndim-1])
+ *                       complex values; F_c(r) = prod_d p_{c,d}[i_d].
+ * envelope: n_env complex FP64 samples at the times gpe_pump_times() lists
+ * for the next gpe_solve; n_env == 0 means envelope == 1 (static pump). */
+int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, const double* envelope,
+                 int64_t n_env);
+
+/* Position noise -i sqrt(dt/2) eta_c xi_c per half-step with constant eta
+ * (position_noise_func, src/kernels.jl:42) and xi drawn in-kernel from
+ * Philox4x32-10 with key `seed`; `call0` is the counter of the next draw (two
+ * draws per step).  kind GPE_NOISE_NONE disables noise. */
+int gpe_set_noise(gpe_handle* h, int32_t kind, const double* eta, uint64_t seed, uint64_t call0);
+
+/* resolve_fixed_timestepping (src/fixed_time_stepping.jl:14-24). */
+int gpe_resolve_timestepping(double dt, double t0, double t1, int64_t nsaves, double* dt_out,
+                             int64_t* steps_per_save);
+
+/* The times at which solve evaluates the pump (src/strang_splitting.jl:58,
+ * 81, 87, 89 with t accumulated as in src/fixed_time_stepping.jl:44):
+ * times[0] = t0, and for step s = 1..S (t_s = t0 + dt + ... + dt, s terms)
+ * times[2s-1] = t_s + dt/2, times[2s] = t_s + dt.  n must be 1 + 2*S. */
+int gpe_pump_times(double t0, double t1, double dt, int64_t nsaves, double* times, int64_t n);
+
+/* The whole fixed-step solve: init + solve! (src/strang_splitting.jl:32-67,
+ * src/fixed_time_stepping.jl:26-54, src/strang_splitting.jl:86-90).
+ *   u0[c]     : M pointers, field arrays (prod(n) * batch complex values)
+ *   result[c] : M pointers, arrays of shape (n..., batch, nsaves + save_start)
+ *   ts        : nsaves + save_start doubles (may be NULL)
+ * Returns after result and ts are written. */
+int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double dt,
+              int64_t nsaves, int32_t save_start, void* const* result, double* ts);
+
+/* Copy the field after the last solve (real space) into u[c]. */
+int gpe_get_field(gpe_handle* h, void* const* u);
+
+/* Device time in ms of the step loop of the last gpe_solve / gpe_advance,
+ * from HIP events recorded on the handle's stream around the loop. */
+int gpe_last_solve_ms(gpe_handle* h, float* ms);
+
+/* Continue the last solve's configuration `nsteps` more steps from the
+ * current field (envelope frozen at its last sample), no frame output.
+ * *ms (may be NULL) receives the device time of the loop. */
+int gpe_advance(gpe_handle* h, int64_t nsteps, float* ms);
+
+/* As gpe_advance, with HIP events around every kernel launch: *ms_x is the
+ * mean duration in ms of the fused real-space kernel (2D x-pass, or the 1D
+ * whole-step kernel), *ms_y that of the 2D k-space y-pass (0 in 1D). */
+int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPE_HIP_H */

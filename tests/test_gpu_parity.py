@@ -1,0 +1,217 @@
+"""GPU parity: libgpe_hip (through the C ABI, via the host mirror) against the
+FP64 oracle on identical inputs.
+
+Bars: FP64 fields -> relative L2 <= 1e-12 on every saved frame; FP32 fields
+-> relative L2 against the FP64 oracle <= 1e-3 (the measured value of each
+case is recorded in gpurun_out/parity_errors.jsonl and stated in the docs); ts
+bit-identical.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-3
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def run_both(ggp, orc, u0, lengths, tspan, dt, nsaves, dtype=np.complex128, save_start=True, seed=None, **terms):
+    ts_o, res_o, _ = orc.solve(orc.Problem(u0, lengths, noise_seed=seed or 0, **_oracle_terms(terms)), tspan, dt,
+                               nsaves, save_start)
+    gu0 = tuple(np.asfortranarray(x.astype(dtype)) for x in u0)
+    gterms = dict(terms)
+    if "noise_prototype" in gterms:
+        gterms["noise_prototype"] = tuple(np.empty(x.shape, gterms.pop("noise_dtype", np.complex128)) for x in u0)
+    else:
+        gterms.pop("noise_dtype", None)
+    rng = ggp.PhiloxRNG(seed or 0) if seed is not None else None
+    ts, res = ggp.solve(ggp.GrossPitaevskiiProblem(gu0, lengths, **gterms), ggp.StrangSplitting(), tspan, dt=dt,
+                        nsaves=nsaves, save_start=save_start, rng=rng)
+    return ts_o, res_o, ts, res
+
+
+def _oracle_terms(terms):
+    t = dict(terms)
+    if "noise_prototype" in t:
+        t["noise_prototype"] = "real" if np.dtype(t.pop("noise_dtype", np.complex128)).kind == "f" else "complex"
+    else:
+        t.pop("noise_dtype", None)
+    return t
+
+
+def check(ts_o, res_o, ts, res, tol):
+    assert np.array_equal(np.asarray(ts), np.asarray(ts_o))
+    worst = 0.0
+    for c in range(len(res_o)):
+        assert res[c].shape == res_o[c].shape
+        for f in range(res_o[c].shape[-1]):
+            worst = max(worst, rel_l2(res[c][..., f].astype(np.complex128), res_o[c][..., f]))
+    _record(worst, res[0].dtype)
+    assert worst <= tol, f"rel L2 {worst:.3e} > {tol:.1e}"
+    return worst
+
+
+def _record(err, dtype):
+    import json
+    import os
+
+    os.makedirs("gpurun_out", exist_ok=True)
+    name = os.environ.get("PYTEST_CURRENT_TEST", "?").split(" ")[0]
+    with open("gpurun_out/parity_errors.jsonl", "a") as f:
+        f.write(json.dumps({"test": name, "dtype": str(dtype), "rel_l2": err}) + "\n")
+
+
+# ------------------------------------------------------------ C1: README case
+def c1_inputs(N=128):
+    L = 8.0
+    rs = np.arange(N) * (L / N)
+    u0 = (np.exp(-(rs[:, None] - 4) ** 2 - (rs[None, :] - 4) ** 2).astype(complex),)
+    return u0, (L, L)
+
+
+def disp_kin(k, p):
+    return sum(x ** 2 for x in k) / 2
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, FP64_TOL), (np.complex64, FP32_TOL)])
+def test_c1_readme_kerr(ggp, orc, dtype, tol):
+    """128^2, L=8, D=|k|^2/2, G=-6|u|^2, dt 0.01 -> 0.00625, 64 frames."""
+    u0, lengths = c1_inputs()
+    out = run_both(ggp, orc, u0, lengths, (0, 0.4), 0.01, 64, dtype=dtype, dispersion=disp_kin,
+                   nonlinearity=lambda u, p: p["g"] * abs(u[0]) ** 2, param={"g": -6.0})
+    err = check(*out, tol)
+    print(f"C1 {np.dtype(dtype).name}: rel L2 {err:.3e}")
+
+
+# --------------------------------------------- two components, 2x2 dispersion
+def polariton_terms(L, A=1.0, w=None):
+    w = w or L / 4
+
+    def dispersion(k, p):
+        kk = sum(x ** 2 for x in k)
+        return [[p["hb"] * kk / (2 * p["m"]) - p["dc"] - 1j * p["gc"], p["W"]],
+                [p["W"], -p["dx"] - 1j * p["gx"]]]
+
+    def nonlinearity(u, p):
+        return [0 * u[0], p["g"] * abs(u[1]) ** 2]
+
+    def pump(r, p, t):
+        f = p["A"] * np.exp(-sum((x - p["L"] / 2) ** 2 for x in r) / p["w"] ** 2)
+        return [f, 0 * f]
+
+    param = dict(hb=0.6582, m=0.18194, dc=0.0, dx=0.0, gc=0.05, gx=0.005, W=2.5, g=0.015, A=A, w=w, L=L)
+    return dict(dispersion=dispersion, nonlinearity=nonlinearity, pump=pump, param=param)
+
+
+@pytest.mark.parametrize("N", [64, 128])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, FP64_TOL), (np.complex64, FP32_TOL)])
+def test_two_component_pumped(ggp, orc, N, dtype, tol):
+    L = 100.0 * N / 512
+    u0 = (np.zeros((N, N), complex), np.zeros((N, N), complex))
+    out = run_both(ggp, orc, u0, (L, L), (0, 10.0), 0.05, 10, dtype=dtype, **polariton_terms(L))
+    err = check(*out, tol)
+    print(f"two-component {N}^2 {np.dtype(dtype).name}: rel L2 {err:.3e}")
+
+
+# --------------------------------------------------- 1D, time-dependent pump
+def test_1d_ramped_pump(ggp, orc):
+    """Bistability-style 1D problem with the pump ramp (pump times follow the
+    reference's t+dt/2, t+dt after t += dt)."""
+    N, L = 256, 256.0
+    p = dict(g=0.01, d=0.3, gam=0.1, w0=1483.0, kz=27.0, Imax=0.6, width=50.0, tmax=60.0, L=L)
+
+    def disp(k, p):
+        return -1j * p["gam"] / 2 + p["w0"] * (1 + k[0] ** 2 / (2 * p["kz"] ** 2)) - (p["w0"] + p["d"])
+
+    def I(t):
+        v = -p["Imax"] * t * (t - p["tmax"]) * 4 / p["tmax"] ** 2
+        return np.maximum(v, 0.0)
+
+    def pump(x, p, t):
+        return np.exp(-((x[0] - p["L"] / 2) ** 2) / p["width"] ** 2) * np.sqrt(I(t))
+
+    u0 = (np.zeros(N, complex),)
+    out = run_both(ggp, orc, u0, (L,), (0, 60), 0.05, 12, dispersion=disp,
+                   nonlinearity=lambda u, p: p["g"] * abs(u[0]) ** 2, pump=pump, param=p)
+    check(*out, FP64_TOL)
+
+
+# ------------------------------------------------------- every FFT length
+@pytest.mark.parametrize("N", [16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+def test_1d_sizes_batched(ggp, orc, N):
+    L = float(N) / 8
+    rng = np.random.default_rng(N)
+    B = 3
+    u0 = (rng.standard_normal((N, B)) + 1j * rng.standard_normal((N, B)),)
+    out = run_both(ggp, orc, u0, (L,), (0, 0.5), 0.05, 2, dispersion=disp_kin,
+                   nonlinearity=lambda u, p: 0.3 * abs(u[0]) ** 2)
+    check(*out, FP64_TOL)
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (32, 64), (64, 32), (256, 16), (16, 512), (1024, 32)])
+def test_2d_shapes(ggp, orc, shape):
+    rng = np.random.default_rng(sum(shape))
+    u0 = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape),)
+    lengths = (shape[0] / 4.0, shape[1] / 2.0)
+    out = run_both(ggp, orc, u0, lengths, (0, 0.2), 0.02, 2, dispersion=disp_kin,
+                   nonlinearity=lambda u, p: 0.1 * abs(u[0]) ** 2)
+    check(*out, FP64_TOL)
+
+
+def test_2d_batched_two_component(ggp, orc):
+    rng = np.random.default_rng(7)
+    shape = (32, 32, 3)
+    u0 = tuple(rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(2))
+    out = run_both(ggp, orc, u0, (8.0, 8.0), (0, 0.4), 0.05, 2, **polariton_terms(8.0))
+    check(*out, FP64_TOL)
+
+
+# -------------------------------------------------------------- noise (TWA)
+@pytest.mark.parametrize("noise_dtype", [np.complex128, np.float64])
+def test_noise_1d_philox(ggp, orc, noise_dtype):
+    N, B, L = 64, 8, 20.0
+    gam = 0.1
+    u0 = (np.zeros((N, B), complex),)
+    out = run_both(ggp, orc, u0, (L,), (0, 4.0), 0.1, 4, seed=1234,
+                   dispersion=lambda k, p: -1j * gam / 2 + k[0] ** 2 / 2,
+                   position_noise_func=lambda u, r, p: math.sqrt(gam / 2 / (L / N)),
+                   noise_prototype=True, noise_dtype=noise_dtype)
+    check(*out, FP64_TOL)
+
+
+def test_noise_2d_batched_kerr(ggp, orc):
+    N, B, L = 32, 4, 8.0
+    u0 = (np.full((N, N, B), 0.5 + 0j),)
+    out = run_both(ggp, orc, u0, (L, L), (0, 1.0), 0.05, 2, seed=99,
+                   dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2 - 0.5 - 0.1j,
+                   nonlinearity=lambda u, p: abs(u[0]) ** 2, pump=lambda r, p, t: 1.2 + 0 * r[0],
+                   position_noise_func=lambda u, r, p: 0.3, noise_prototype=True)
+    check(*out, FP64_TOL)
+
+
+# ---------------------------------------------------------------- potential
+@pytest.mark.parametrize("ncomp", [1, 2])
+def test_potential(ggp, orc, ncomp):
+    N, L = 64, 10.0
+    rng = np.random.default_rng(3)
+    u0 = tuple(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)) for _ in range(ncomp))
+
+    def pot(r, p):
+        v = 0.5 * ((r[0] - 5) ** 2 + (r[1] - 5) ** 2)
+        return v if ncomp == 1 else [v, 2 * v - 0.01j]
+
+    out = run_both(ggp, orc, u0, (L, L), (0, 0.3), 0.01, 3, dispersion=disp_kin, potential=pot)
+    check(*out, FP64_TOL)
+
+
+def test_save_start_false_and_frames(ggp, orc):
+    u0, lengths = c1_inputs(64)
+    out = run_both(ggp, orc, u0, lengths, (0, 0.2), 0.01, 5, save_start=False, dispersion=disp_kin)
+    check(*out, FP64_TOL)
