@@ -50,7 +50,9 @@ struct gpe_handle {
   void* tw = nullptr;
   int64_t ntw = 0;
   // dispersion
-  int disp_form = GPE_FORM_NONE;
+  int disp_form = GPE_FORM_NONE;  // device form: 1 scalar, 2 diag, 3 full, 4 symmetric 2x2
+  int ed_fold = 0;                // bit d: D is even in k_d and the table is folded
+  int64_t ed_e0 = 0, ed_e1 = 0;   // compact table extents
   double* dD = nullptr;
   void* ed = nullptr;
   double ed_dt = NAN;
@@ -60,7 +62,9 @@ struct gpe_handle {
   void* ev[kMaxM] = {nullptr, nullptr};
   double ev_dt = NAN;
   // nonlinearity
-  bool has_g = false;
+  int gmask = 0;
+  int g_cplx = 0;
+  int g_coupled = 0;
   double g[kMaxM][kMaxM][2] = {};
   double c[kMaxM][2] = {};
   // pump
@@ -243,6 +247,10 @@ extern "C" int gpe_destroy(gpe_handle* h) {
 
 static int per_point(int form, int M) { return form == GPE_FORM_SCALAR ? 1 : (form == GPE_FORM_DIAG ? M : M * M); }
 
+// D(k) is uploaded in a compact form: a 1x1 matrix or 1-vector becomes the
+// scalar form, a 2x2 matrix with D01 == D10 everywhere is stored as (a, b, d),
+// and when D(k) == D(-k_d) exactly along dimension d only k_d in [0, N_d/2]
+// is stored.
 extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) {
   if (!h) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: null handle");
   HIPCHK(hipSetDevice(h->d.device));
@@ -253,16 +261,67 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
   }
   if (form < GPE_FORM_SCALAR || form > GPE_FORM_FULL) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: form=%d", form);
   if (!D) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: null table");
-  const size_t bytes = (size_t)h->nsp * per_point(form, h->M) * 2 * sizeof(double);
+  const int M = h->M;
+  const int per_in = per_point(form, M);
+  const int64_t n0 = h->n0, n1 = h->n1;
+  // device form and values per entry
+  int dform = form;
+  if (M == 1) dform = GPE_FORM_SCALAR;
+  if (dform == GPE_FORM_FULL) {
+    bool sym = true;
+    for (int64_t k = 0; k < h->nsp && sym; ++k) {
+      const double* e = D + 2 * (size_t)k * 4;
+      sym = e[2] == e[4] && e[3] == e[5];
+    }
+    if (sym) dform = 4;
+  }
+  auto entry = [&](int64_t k0, int64_t k1) { return D + 2 * (size_t)(k0 + n0 * k1) * per_in; };
+  auto same = [&](const double* x, const double* y) {
+    for (int i = 0; i < 2 * per_in; ++i)
+      if (x[i] != y[i]) return false;
+    return true;
+  };
+  int fold = 0;
+  bool even0 = true, even1 = (h->ndim == 2);
+  for (int64_t k1 = 0; k1 < n1 && (even0 || even1); ++k1)
+    for (int64_t k0 = 0; k0 < n0 && (even0 || even1); ++k0) {
+      if (even0 && !same(entry(k0, k1), entry((n0 - k0) % n0, k1))) even0 = false;
+      if (even1 && !same(entry(k0, k1), entry(k0, (n1 - k1) % n1))) even1 = false;
+    }
+  if (even0) fold |= 1;
+  if (even1) fold |= 2;
+  const int64_t e0 = even0 ? n0 / 2 + 1 : n0, e1 = even1 ? n1 / 2 + 1 : n1;
+  const int per_out = dform == GPE_FORM_SCALAR ? 1 : (dform == GPE_FORM_DIAG ? M : (dform == GPE_FORM_FULL ? 4 : 3));
+  std::vector<double> compact((size_t)e0 * e1 * per_out * 2);
+  for (int64_t k1 = 0; k1 < e1; ++k1)
+    for (int64_t k0 = 0; k0 < e0; ++k0) {
+      const double* src = entry(k0, k1);
+      double* dst = compact.data() + 2 * (size_t)(k0 + e0 * k1) * per_out;
+      if (dform == 4) {
+        const int pick[3] = {0, 1, 3};
+        for (int i = 0; i < 3; ++i) {
+          dst[2 * i] = src[2 * pick[i]];
+          dst[2 * i + 1] = src[2 * pick[i] + 1];
+        }
+      } else if (dform == GPE_FORM_SCALAR && per_in != 1) {
+        dst[0] = src[0];
+        dst[1] = src[1];
+      } else {
+        for (int i = 0; i < 2 * per_out; ++i) dst[i] = src[i];
+      }
+    }
   void* p = h->dD;
   dfree(p);
   h->dD = nullptr;
   dfree(h->ed);
-  int rc = upload(&p, D, bytes);
+  int rc = upload(&p, compact.data(), compact.size() * sizeof(double));
   h->dD = (double*)p;
   if (rc) return rc;
-  HIPCHK(hipMalloc(&h->ed, (size_t)h->nsp * per_point(form, h->M) * h->csz));
-  h->disp_form = form;
+  HIPCHK(hipMalloc(&h->ed, (size_t)e0 * e1 * per_out * h->csz));
+  h->disp_form = dform;
+  h->ed_fold = fold;
+  h->ed_e0 = e0;
+  h->ed_e1 = e1;
   h->ed_dt = NAN;
   return GPE_OK;
 }
@@ -293,12 +352,27 @@ extern "C" int gpe_set_potential(gpe_handle* h, int32_t form, const double* V) {
 extern "C" int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double* c) {
   if (!h) return fail(GPE_ERR_INVALID, "gpe_set_nonlinearity: null handle");
   const int M = h->M;
-  h->has_g = (g != nullptr) || (c != nullptr);
+  h->gmask = 0;
+  h->g_cplx = 0;
   for (int i = 0; i < M; ++i) {
+    bool any = false;
     for (int j = 0; j < M; ++j)
-      for (int r = 0; r < 2; ++r) h->g[i][j][r] = g ? g[2 * (i + M * j) + r] : 0.0;
-    for (int r = 0; r < 2; ++r) h->c[i][r] = c ? c[2 * i + r] : 0.0;
+      for (int r = 0; r < 2; ++r) {
+        h->g[i][j][r] = g ? g[2 * (i + M * j) + r] : 0.0;
+        any = any || h->g[i][j][r] != 0.0;
+        if (r == 1 && h->g[i][j][r] != 0.0) h->g_cplx = 1;
+      }
+    for (int r = 0; r < 2; ++r) {
+      h->c[i][r] = c ? c[2 * i + r] : 0.0;
+      any = any || h->c[i][r] != 0.0;
+      if (r == 1 && h->c[i][r] != 0.0) h->g_cplx = 1;
+    }
+    if (any) h->gmask |= 1 << i;
   }
+  h->g_coupled = 0;
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < M; ++j)
+      if (i != j && (h->g[i][j][0] != 0.0 || h->g[i][j][1] != 0.0)) h->g_coupled = 1;
   return GPE_OK;
 }
 
@@ -389,12 +463,12 @@ static int prepare_tables(gpe_handle* h, double dt) {
     EdArgs a;
     a.D = h->dD;
     a.out = h->ed;
-    a.npts = h->nsp;
+    a.npts = h->ed_e0 * h->ed_e1;
     a.form = h->disp_form;
     a.M = h->M;
     a.dt = dt;
     a.scale = 1.0 / (double)h->nsp;
-    const unsigned blocks = (unsigned)((h->nsp + 255) / 256);
+    const unsigned blocks = (unsigned)((a.npts + 255) / 256);
     if (h->f64)
       hipLaunchKernelGGL(ed_table_kernel<double>, dim3(blocks), dim3(256), 0, h->stream, a);
     else
@@ -424,7 +498,9 @@ static int prepare_tables(gpe_handle* h, double dt) {
 static PointTerms point_terms(const gpe_handle* h) {
   PointTerms pt;
   memset(&pt, 0, sizeof pt);
-  pt.has_g = h->has_g;
+  pt.gmask = h->gmask;
+  pt.g_cplx = h->g_cplx;
+  pt.g_coupled = h->g_coupled;
   memcpy(pt.g, h->g, sizeof pt.g);
   memcpy(pt.c, h->c, sizeof pt.c);
   pt.pump_kind = h->pump_kind;
@@ -480,6 +556,19 @@ static XPassArgs x_args(const gpe_handle* h) {
   return a;
 }
 
+static EdDesc ed_desc(const gpe_handle* h) {
+  EdDesc d;
+  memset(&d, 0, sizeof d);
+  d.form = h->disp_form == GPE_FORM_NONE ? 0 : h->disp_form;
+  d.fold = h->ed_fold;
+  d.tab = h->ed;
+  d.n0 = h->n0;
+  d.n1 = h->n1;
+  d.e0 = h->ed_e0;
+  d.scale = 1.0 / (double)h->nsp;
+  return d;
+}
+
 static YPassArgs y_args(const gpe_handle* h) {
   YPassArgs a;
   memset(&a, 0, sizeof a);
@@ -489,9 +578,7 @@ static YPassArgs y_args(const gpe_handle* h) {
   a.ncolblk = h->n0 / C;
   a.nblocks = a.ncolblk * h->batch;
   a.xcd_remap = (a.nblocks % 8 == 0) ? 1 : 0;
-  a.ed_form = h->disp_form == GPE_FORM_NONE ? 0 : h->disp_form;
-  a.ed = h->ed;
-  a.scale = 1.0 / (double)h->nsp;
+  a.ed = ed_desc(h);
   a.tw = h->tw;
   a.tws = (int)(h->ntw / h->n1);
   return a;
@@ -503,9 +590,7 @@ static Step1DArgs s1_args(gpe_handle* h, double dt) {
   for (int c = 0; c < kMaxM; ++c) a.u[c] = h->u[c];
   a.nlines = h->batch;
   a.delta = dt / 2;
-  a.ed_form = h->disp_form == GPE_FORM_NONE ? 0 : h->disp_form;
-  a.ed = h->ed;
-  a.scale = 1.0 / (double)h->nsp;
+  a.ed = ed_desc(h);
   a.tw = h->tw;
   a.tws = (int)(h->ntw / h->n0);
   a.pt = point_terms(h);

@@ -129,44 +129,69 @@ struct PassRadix {
   static constexpr int value = (rem % 8 == 0) ? 8 : rem;
 };
 
-// Number of LDS exchanges (passes - 1) for length N.
-__host__ __device__ constexpr int n_exchanges(int N) {
+// Twiddles a thread needs over all passes of a length-N transform (passes
+// with NS > 1 carry R-1 twiddles per butterfly), and the offset of pass NS.
+__host__ __device__ constexpr int tw_count_before(int N, int NS) {
   int ns = 1, c = 0;
-  while (ns < N) {
-    int rem = N / ns;
-    int r = (rem % 8 == 0) ? 8 : rem;
+  while (ns < N && ns < NS) {
+    const int rem = N / ns;
+    const int r = (rem % 8 == 0) ? 8 : rem;
+    if (ns > 1) c += (kE / r) * (r - 1);
     ns *= r;
-    ++c;
   }
-  return c - 1;
+  return c;
+}
+__host__ __device__ constexpr int tw_count(int N) { return tw_count_before(N, N) > 0 ? tw_count_before(N, N) : 1; }
+
+// Per-thread twiddle registers of one transform length, loaded once (at
+// kernel start, together with the field) from the FP64-accurate table
+// tw[m] = exp(-2 pi i m / NT), NT = N * tws.  The inverse transform uses the
+// conjugates.
+template <typename R, int N>
+struct TwReg {
+  Cx<R> w[tw_count(N)];
+};
+
+template <typename R, int N, int T, int NS>
+__device__ __forceinline__ void load_twiddles_pass(TwReg<R, N>& tr, int t, const Cx<R>* __restrict__ tw, int tws) {
+  constexpr int RADIX = PassRadix<N, NS>::value;
+  constexpr int B = kE / RADIX;
+  if constexpr (NS > 1) {
+    constexpr int off = tw_count_before(N, NS);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int k = (t + b * T) & (NS - 1);
+      const int base = k * (N / (NS * RADIX)) * tws;
+#pragma unroll
+      for (int r = 1; r < RADIX; ++r) tr.w[off + b * (RADIX - 1) + r - 1] = tw[r * base];
+    }
+  }
+  if constexpr (NS * RADIX < N) load_twiddles_pass<R, N, T, NS * RADIX>(tr, t, tw, tws);
+}
+
+template <typename R, int N, int T>
+__device__ __forceinline__ void load_twiddles(TwReg<R, N>& tr, int t, const Cx<R>* __restrict__ tw, int tws) {
+  load_twiddles_pass<R, N, T, 1>(tr, t, tw, tws);
 }
 
 // One Stockham pass on M bundled lines followed (unless last) by an LDS
 // exchange into the cyclic layout.  `lds` points at this thread-group's lines;
 // line c of the bundle lives at lds + c*lstride (+ lds_pad(p)).  Two LDS
 // buffers (lds, lds + buf_off) alternate between passes so one barrier per
-// pass suffices.  tw is the FP64-accurate table exp(-2 pi i m / NT), NT = N*tws.
+// pass suffices.
 template <typename R, int N, int T, int M, bool INV, int NS, int BUF>
 __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int lstride, int buf_off, int t,
-                                              const Cx<R>* __restrict__ tw, int tws) {
+                                              const TwReg<R, N>& tr) {
   constexpr int RADIX = PassRadix<N, NS>::value;
   constexpr int B = kE / RADIX;
   constexpr bool LAST = (NS * RADIX == N);
+  constexpr int off = tw_count_before(N, NS);
   static_assert(kE % RADIX == 0, "radix must divide the per-thread count");
   Cx<R>* buf = lds + BUF * buf_off;
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int j = t + b * T;
     const int k = j & (NS - 1);
-    Cx<R> w[RADIX];
-    if constexpr (NS > 1) {
-      const int base = k * (N / (NS * RADIX)) * tws;
-#pragma unroll
-      for (int r = 1; r < RADIX; ++r) {
-        Cx<R> x = tw[r * base];
-        w[r] = INV ? conjx(x) : x;
-      }
-    }
 #pragma unroll
     for (int c = 0; c < M; ++c) {
       Cx<R> a[RADIX];
@@ -174,7 +199,10 @@ __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int
       for (int r = 0; r < RADIX; ++r) a[r] = v[c][b + r * B];
       if constexpr (NS > 1) {
 #pragma unroll
-        for (int r = 1; r < RADIX; ++r) a[r] = a[r] * w[r];
+        for (int r = 1; r < RADIX; ++r) {
+          const Cx<R> w = tr.w[off + b * (RADIX - 1) + r - 1];
+          a[r] = a[r] * (INV ? conjx(w) : w);
+        }
       }
       dft<R, INV, RADIX>(a);
       if constexpr (LAST) {
@@ -193,7 +221,7 @@ __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int
     for (int c = 0; c < M; ++c)
 #pragma unroll
       for (int m = 0; m < kE; ++m) v[c][m] = buf[c * lstride + lds_pad(t + m * T)];
-    stockham_pass<R, N, T, M, INV, NS * RADIX, 1 - BUF>(v, lds, lstride, buf_off, t, tw, tws);
+    stockham_pass<R, N, T, M, INV, NS * RADIX, 1 - BUF>(v, lds, lstride, buf_off, t, tr);
   }
 }
 
@@ -201,9 +229,9 @@ __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int
 // of M bundled lines, input and output in the cyclic register layout.
 template <typename R, int N, int T, int M, bool INV>
 __device__ __forceinline__ void fft_lines(Cx<R> (&v)[M][kE], Cx<R>* lds, int lstride, int buf_off, int t,
-                                          const Cx<R>* __restrict__ tw, int tws) {
+                                          const TwReg<R, N>& tr) {
   static_assert(N / T == kE, "E = N/T must be 8");
-  stockham_pass<R, N, T, M, INV, 1, 0>(v, lds, lstride, buf_off, t, tw, tws);
+  stockham_pass<R, N, T, M, INV, 1, 0>(v, lds, lstride, buf_off, t, tr);
 }
 
 // ------------------------------------------------------------------- Philox
@@ -228,12 +256,15 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-// Box-Muller: (wa, wb) -> complex normal with E|z|^2 = 1.
+// Box-Muller: (wa, wb) -> complex normal with E|z|^2 = 1:
+// z = sqrt(-log u_a) exp(2 pi i u_b), u = (w + 1/2) 2^-32.  The FP64 form is
+// kept out of line (library log and sincospi); the FP32 form uses the
+// hardware log2 and the revolution-based sin/cos of the ALU.
 template <typename R>
-__device__ __forceinline__ Cx<R> box_muller(uint32_t wa, uint32_t wb);
+__device__ Cx<R> box_muller(uint32_t wa, uint32_t wb);
 
 template <>
-__device__ __forceinline__ Cx<double> box_muller<double>(uint32_t wa, uint32_t wb) {
+__device__ __noinline__ Cx<double> box_muller<double>(uint32_t wa, uint32_t wb) {
   const double ua = ((double)wa + 0.5) * 2.3283064365386962890625e-10;
   const double ub = ((double)wb + 0.5) * 2.3283064365386962890625e-10;
   const double r = sqrt(-log(ua));
@@ -246,10 +277,8 @@ template <>
 __device__ __forceinline__ Cx<float> box_muller<float>(uint32_t wa, uint32_t wb) {
   const float ua = ((float)(wa >> 8) + 0.5f) * 5.9604644775390625e-08f;
   const float ub = ((float)(wb >> 8) + 0.5f) * 5.9604644775390625e-08f;
-  const float r = sqrtf(-logf(ua));
-  float s, c;
-  sincospif(2.0f * ub, &s, &c);
-  return mk<float>(r * c, r * s);
+  const float r = sqrtf(-__logf(ua));
+  return mk<float>(r * __builtin_amdgcn_cosf(ub), r * __builtin_amdgcn_sinf(ub));
 }
 
 }  // namespace gpe

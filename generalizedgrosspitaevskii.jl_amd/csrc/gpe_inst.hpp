@@ -14,11 +14,11 @@ template <typename R, int N, int M>
 static hipError_t launch_x(const XPassArgs& a, int64_t nlines, hipStream_t s) {
   constexpr int W = x_lines_per_wg(N);
   constexpr int T = N / kE;
-  const size_t lds = 2ull * M * W * lds_line_stride(N) * sizeof(Cx<R>);
-  auto k = x_pass_kernel<R, N, M, W>;
+  const size_t lds = xlds_bytes<R, N, M, W>();
+  auto k = a.pt.noise_kind ? x_pass_kernel<R, N, M, W, true> : x_pass_kernel<R, N, M, W, false>;
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3((unsigned)((nlines + W - 1) / W)), dim3(W * T), lds, s, a);
+  hipLaunchKernelGGL(k, dim3((unsigned)((nlines + W - 1) / W)), dim3(M * W * T), lds, s, a);
   return hipGetLastError();
 }
 
@@ -26,11 +26,11 @@ template <typename R, int N, int M>
 static hipError_t launch_y(const YPassArgs& a, hipStream_t s) {
   constexpr int T = N / kE;
   constexpr int C = y_cols_per_wg(N);
-  const size_t lds = 2ull * M * C * lds_line_stride(N) * sizeof(Cx<R>);
+  const size_t lds = ylds_bytes<R, N, M, C>();
   auto k = y_pass_kernel<R, N, M, C>;
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3((unsigned)a.nblocks), dim3(C * T), lds, s, a);
+  hipLaunchKernelGGL(k, dim3((unsigned)a.nblocks), dim3(M * C * T), lds, s, a);
   return hipGetLastError();
 }
 
@@ -38,11 +38,11 @@ template <typename R, int N, int M>
 static hipError_t launch_1d(const Step1DArgs& a, hipStream_t s) {
   constexpr int W = x_lines_per_wg(N);
   constexpr int T = N / kE;
-  const size_t lds = 2ull * M * W * lds_line_stride(N) * sizeof(Cx<R>);
-  auto k = step1d_kernel<R, N, M, W>;
+  const size_t lds = xlds_bytes<R, N, M, W>();
+  auto k = a.pt.noise_kind ? step1d_kernel<R, N, M, W, true> : step1d_kernel<R, N, M, W, false>;
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3((unsigned)((a.nlines + W - 1) / W)), dim3(W * T), lds, s, a);
+  hipLaunchKernelGGL(k, dim3((unsigned)((a.nlines + W - 1) / W)), dim3(M * W * T), lds, s, a);
   return hipGetLastError();
 }
 

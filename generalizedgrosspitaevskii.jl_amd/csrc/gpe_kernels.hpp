@@ -18,6 +18,7 @@
 //     F_c(r, tau) = profile_c(r) * envelope(tau).
 #pragma once
 #include "gpe_device.hpp"
+#include "gpe_math.hpp"
 
 namespace gpe {
 
@@ -33,7 +34,9 @@ struct HalfStep {
 
 // Per-point terms shared by every real-space half-step.
 struct PointTerms {
-  int has_g;  // nonlinearity G_c = c_c + sum_j g[c][j] |u_j|^2 (complex)
+  int gmask;   // bit c: component c has a nonlinearity G_c = c_c + sum_j g[c][j] |u_j|^2
+  int g_cplx;  // some coefficient of G has an imaginary part (gain/loss)
+  int g_coupled;  // some G_c depends on |u_j|^2 with j != c
   double g[kMaxM][kMaxM][2];
   double c[kMaxM][2];
   int pump_kind;                 // 0 none, 1 table, 2 separable
@@ -55,137 +58,237 @@ __device__ __forceinline__ Cx<R> cxd(const double (&a)[2]) {
   return mk<R>((R)a[0], (R)a[1]);
 }
 
-// The Wiener increments -i sqrt(delta) eta_c xi_c of a thread's M x E values.
+// Each thread works on ONE component: the M components of a line (or a
+// column) are held by M groups of threads of the same workgroup, and values
+// cross between components through LDS only where the arithmetic couples
+// them (|u_j|^2 in G_c for j != c; the off-diagonal of exp(-i dt D)).
+// Workgroups are laid out component-major, so every wavefront belongs to a
+// single component and component-dependent branches are wave-uniform.
+
+__device__ __forceinline__ const void* selp(int c, const void* a0, const void* a1) { return c ? a1 : a0; }
+
+// The Wiener increment -i sqrt(delta) eta_c xi_c of the thread's E values.
 // Element m and m + E/2 (x and x + N/2 on the line) share one Philox draw:
 // ctr = (P mod 2^32, c + 16 (P >> 32), call mod 2^32, call >> 32) with
 // P = line * N/2 + x, key = seed; words (x,y) feed x, (z,w) feed x + N/2.
-template <typename R, int N, int M>
-__device__ __forceinline__ void noise_terms(Cx<R> (&nz)[M][kE], const HalfStep& hs, const PointTerms& pt, int t,
-                                            int T, int64_t line) {
-  const R sq = (R)sqrt(hs.delta);
+template <typename R, int N>
+__device__ __forceinline__ void noise_terms(Cx<R> (&nz)[kE], const HalfStep& hs, const PointTerms& pt, int c, R eta,
+                                            int t, int T, int64_t line) {
+  const R a = (R)sqrt(hs.delta) * eta;
 #pragma unroll
   for (int m = 0; m < kE / 2; ++m) {
     const uint64_t P = (uint64_t)line * (uint64_t)(N / 2) + (uint64_t)(t + m * T);
-#pragma unroll
-    for (int c = 0; c < M; ++c) {
-      U4 ctr;
-      ctr.x = (uint32_t)P;
-      ctr.y = (uint32_t)c + 16u * (uint32_t)(P >> 32);
-      ctr.z = (uint32_t)hs.call;
-      ctr.w = (uint32_t)(hs.call >> 32);
-      const U4 w = philox4x32_10(ctr, pt.seed_lo, pt.seed_hi);
-      Cx<R> z0 = box_muller<R>(w.x, w.y);
-      Cx<R> z1 = box_muller<R>(w.z, w.w);
-      if (pt.noise_kind == 2) {
-        const R s2 = (R)1.41421356237309504880168872420969808;
-        z0 = mk<R>(z0.re * s2, (R)0);
-        z1 = mk<R>(z1.re * s2, (R)0);
-      }
-      const R a = sq * (R)pt.eta[c];
-      nz[c][m] = mk<R>(a * z0.im, -a * z0.re);
-      nz[c][m + kE / 2] = mk<R>(a * z1.im, -a * z1.re);
+    U4 ctr;
+    ctr.x = (uint32_t)P;
+    ctr.y = (uint32_t)c + 16u * (uint32_t)(P >> 32);
+    ctr.z = (uint32_t)hs.call;
+    ctr.w = (uint32_t)(hs.call >> 32);
+    const U4 w = philox4x32_10(ctr, pt.seed_lo, pt.seed_hi);
+    Cx<R> z0 = box_muller<R>(w.x, w.y);
+    Cx<R> z1 = box_muller<R>(w.z, w.w);
+    if (pt.noise_kind == 2) {
+      const R s2 = (R)1.41421356237309504880168872420969808;
+      z0 = mk<R>(z0.re * s2, (R)0);
+      z1 = mk<R>(z1.re * s2, (R)0);
     }
+    nz[m] = mk<R>(a * z0.im, -a * z0.re);
+    nz[m + kE / 2] = mk<R>(a * z1.im, -a * z1.re);
   }
 }
 
-// Real-space half-step (muladd_kernel!, src/kernels.jl:37-54) on the M x E
-// values a thread holds.  Element m sits at dim-0 index x = t + m*T of global
-// line `line`; `sp_off` is the spatial flat index of x = 0 on that line and
-// `y` its dim-1 index (2D; 0 in 1D).
-template <typename R, int N, int M>
-__device__ __forceinline__ void half_step(Cx<R> (&v)[M][kE], const HalfStep& hs, const PointTerms& pt, int t,
-                                          int T, int64_t line, int64_t y, int64_t sp_off) {
-  const R delta = (R)hs.delta;
-  Cx<R> nz[M][kE];
-  if (pt.noise_kind) noise_terms<R, N, M>(nz, hs, pt, t, T, line);
-  Cx<R> pb[M];
-  if (pt.pump_kind == 2) {
+// Per-thread constants of the real-space half-step for component c: the
+// pump profile and exp(-i dt/2 V) at the thread's points (loaded at kernel
+// start, so their latency overlaps the field loads) and the nonlinearity
+// coefficients of row c in the field precision.
+template <typename R, int M>
+struct PointCache {
+  Cx<R> pump[kE];
+  Cx<R> pot[kE];
+  bool pumped, has_g;
+  R cr, ci, gr[M], gi[M], eta;
+};
+
+template <typename R, int M>
+__device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PointTerms& pt, int c, int t, int T,
+                                                 int64_t y, int64_t sp_off) {
+  const void* pa = selp(c, pt.pump_a[0], pt.pump_a[M - 1]);
+  pc.pumped = pt.pump_kind != 0 && pa != nullptr;
+  if (pc.pumped) {
+    const void* pbp = selp(c, pt.pump_b[0], pt.pump_b[M - 1]);
+    const Cx<R> pb = (pt.pump_kind == 2 && pbp) ? ldc<R>(pbp, y) : mk<R>((R)1, (R)0);
 #pragma unroll
-    for (int c = 0; c < M; ++c) pb[c] = pt.pump_b[c] ? ldc<R>(pt.pump_b[c], y) : mk<R>((R)1, (R)0);
+    for (int m = 0; m < kE; ++m) {
+      const int x = t + m * T;
+      pc.pump[m] = (pt.pump_kind == 1) ? ldc<R>(pa, sp_off + x) : ldc<R>(pa, x) * pb;
+    }
+  }
+  if (pt.pot_kind) {
+    const void* pp = selp(c, pt.pot[0], pt.pot[M - 1]);
+#pragma unroll
+    for (int m = 0; m < kE; ++m) pc.pot[m] = ldc<R>(pp, sp_off + t + m * T);
+  }
+  pc.has_g = (pt.gmask >> c) & 1;
+  pc.cr = (R)(c ? pt.c[M - 1][0] : pt.c[0][0]);
+  pc.ci = (R)(c ? pt.c[M - 1][1] : pt.c[0][1]);
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    pc.gr[j] = (R)(c ? pt.g[M - 1][j][0] : pt.g[0][j][0]);
+    pc.gi[j] = (R)(c ? pt.g[M - 1][j][1] : pt.g[0][j][1]);
+  }
+  pc.eta = (R)(c ? pt.eta[M - 1] : pt.eta[0]);
+}
+
+// Real-space half-step (muladd_kernel!, src/kernels.jl:37-54) on the E values
+// of component c a thread holds; element m sits at dim-0 index x = t + m*T of
+// global line `line` (the Philox layout's line index).  When G couples the
+// components, |u_j|^2 of the other components comes through the LDS slice
+// `a2` (M rows of N values for this thread's line); every thread of the
+// workgroup takes the same branch, so the barriers are uniform.
+template <typename R, int N, int M, bool NOISE>
+__device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep& hs, const PointTerms& pt,
+                                          const PointCache<R, M>& pc, int t, int T, int64_t line, R* a2) {
+  const R delta = (R)hs.delta;
+  Cx<R> nz[kE];
+  if constexpr (NOISE) noise_terms<R, N>(nz, hs, pt, c, pc.eta, t, T, line);
+  R a2j[M][kE];
+  if (pt.gmask) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) a2j[0][m] = abs2(v[m]);
+    if (M > 1 && pt.g_coupled) {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < kE; ++m) a2[c * N + t + m * T] = a2j[0][m];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int m = 0; m < kE; ++m) a2j[j][m] = a2[j * N + t + m * T];
+      __syncthreads();
+    }
   }
   const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
 #pragma unroll
   for (int m = 0; m < kE; ++m) {
-    const int x = t + m * T;
-    const int64_t pidx = sp_off + x;
-    Cx<R> u[M], e[M];
-#pragma unroll
-    for (int c = 0; c < M; ++c) {
-      u[c] = v[c][m];
-      e[c] = mk<R>((R)1, (R)0);
-    }
+    Cx<R> e = mk<R>((R)1, (R)0);
     // cis(-delta G(u)) from the pre-update fields (src/kernels.jl:40-44)
-    if (pt.has_g) {
-      R a2[M];
-#pragma unroll
-      for (int j = 0; j < M; ++j) a2[j] = abs2(u[j]);
-#pragma unroll
-      for (int c = 0; c < M; ++c) {
-        R gr = (R)pt.c[c][0], gi = (R)pt.c[c][1];
+    if (pc.has_g) {
+      R gr = pc.cr, gi = pc.ci;
+      if (M > 1 && pt.g_coupled) {
 #pragma unroll
         for (int j = 0; j < M; ++j) {
-          gr += (R)pt.g[c][j][0] * a2[j];
-          gi += (R)pt.g[c][j][1] * a2[j];
+          gr += pc.gr[j] * a2j[j][m];
+          gi += pc.gi[j] * a2j[j][m];
         }
-        R s, co;
-        if constexpr (sizeof(R) == 8) {
-          sincos(delta * gr, &s, &co);
-        } else {
-          sincosf(delta * gr, &s, &co);
-        }
-        // exp(-i delta (gr + i gi)) = exp(delta gi) (cos(delta gr) - i sin(delta gr))
-        const R mag = (gi == (R)0) ? (R)1 : (R)exp(delta * gi);
-        e[c] = mk<R>(mag * co, -mag * s);
+      } else {
+        const R gcr = (M > 1 && c) ? pc.gr[M - 1] : pc.gr[0];
+        const R gci = (M > 1 && c) ? pc.gi[M - 1] : pc.gi[0];
+        gr += gcr * a2j[0][m];
+        gi += gci * a2j[0][m];
+      }
+      R s, co;
+      fast_sincos(delta * gr, s, co);
+      // exp(-i delta (gr + i gi)) = exp(delta gi) (cos(delta gr) - i sin(delta gr))
+      if (pt.g_cplx) {
+        const R mag = exp(delta * gi);
+        e = mk<R>(mag * co, -mag * s);
+      } else {
+        e = mk<R>(co, -s);
       }
     }
-    if (pt.pot_kind) {
-#pragma unroll
-      for (int c = 0; c < M; ++c) e[c] = e[c] * ldc<R>(pt.pot[c], pidx);
-    }
+    if (pt.pot_kind) e = e * pc.pot[m];
     // e (u + dt/2 F_now) + dt/2 F_next + noise   (src/kernels.jl:47-48)
-#pragma unroll
-    for (int c = 0; c < M; ++c) {
-      Cx<R> w = u[c];
-      Cx<R> fp;
-      const bool pumped = pt.pump_kind != 0 && pt.pump_a[c] != nullptr;
-      if (pumped) {
-        fp = (pt.pump_kind == 1) ? ldc<R>(pt.pump_a[c], pidx) : ldc<R>(pt.pump_a[c], x) * pb[c];
-        w = w + fp * fnow;
-      }
-      w = e[c] * w;
-      if (pumped) w = w + fp * fnext;
-      if (pt.noise_kind) w = w + nz[c][m];
-      v[c][m] = w;
-    }
+    Cx<R> w = v[m];
+    if (pc.pumped) w = w + pc.pump[m] * fnow;
+    w = e * w;
+    if (pc.pumped) w = w + pc.pump[m] * fnext;
+    if constexpr (NOISE) w = w + nz[m];
+    v[m] = w;
   }
 }
 
-// exp(-i dt D(k)) (scaled by 1/prod(N)) applied at flat k index kidx.
-// form 0: scale only (no dispersion), 1: scalar, 2: diagonal, 3: full MxM
-// column-major.
+// exp(-i dt D(k)) / prod(N) of the k points a thread holds.  The table is
+// indexed by (k0', k1') where k_d' = min(k_d, N_d - k_d) when D is even in
+// k_d (fold bit d), giving (N_d/2+1) rows instead of N_d.  Entry forms:
+// 1 scalar, 2 diagonal (M values), 3 full 2x2 (column-major), 4 symmetric
+// 2x2 stored as (a, b, d) for [a b; b d].
+struct EdDesc {
+  int form;       // 0: no dispersion, multiply by `scale` only
+  int fold;       // bit 0: dim 0 folded, bit 1: dim 1 folded
+  const void* tab;
+  int64_t n0, n1; // grid extents
+  int64_t e0;     // table row length along dim 0
+  double scale;
+};
+
+// Row c of the matrix at the thread's k points: own[m] = E_cc (or the scalar
+// or diagonal entry), other[m] = E_c,1-c for the 2x2 forms.
+template <typename R>
+struct EdVals {
+  Cx<R> own[kE];
+  Cx<R> other[kE];
+};
+
+__device__ __forceinline__ int64_t ed_index(const EdDesc& d, int64_t k0, int64_t k1) {
+  if (d.fold & 1) k0 = (k0 <= d.n0 - k0) ? k0 : d.n0 - k0;
+  if (d.fold & 2) k1 = (k1 <= d.n1 - k1) ? k1 : d.n1 - k1;
+  return k0 + d.e0 * k1;
+}
+
+// k points (k0, k1) = (a0 + s0 m, a1 + s1 m), m = 0..E-1.
 template <typename R, int M>
-__device__ __forceinline__ void apply_ed(Cx<R> (&w)[M], int form, const void* ed, int64_t kidx, R scl) {
-  if (form == 0) {
-#pragma unroll
-    for (int c = 0; c < M; ++c) w[c] = scale(w[c], scl);
-  } else if (form == 1) {
-    const Cx<R> e = ldc<R>(ed, kidx);
-#pragma unroll
-    for (int c = 0; c < M; ++c) w[c] = e * w[c];
-  } else if (form == 2) {
-#pragma unroll
-    for (int c = 0; c < M; ++c) w[c] = ldc<R>(ed, kidx * M + c) * w[c];
+__device__ __forceinline__ void load_ed(EdVals<R>& ev, const EdDesc& d, int c, int64_t a0, int64_t s0, int64_t a1,
+                                        int64_t s1) {
+  if (d.form == 0) return;
+  int per, io, ix;
+  if (d.form == 1) {
+    per = 1; io = 0; ix = 0;
+  } else if (d.form == 2) {
+    per = M; io = c; ix = 0;
+  } else if (d.form == 3) {
+    per = 4; io = c ? 3 : 0; ix = c ? 1 : 2;
   } else {
-    Cx<R> o[M];
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-      o[i] = ldc<R>(ed, kidx * M * M + i) * w[0];
-#pragma unroll
-      for (int j = 1; j < M; ++j) o[i] = o[i] + ldc<R>(ed, kidx * M * M + i + M * j) * w[j];
-    }
-#pragma unroll
-    for (int c = 0; c < M; ++c) w[c] = o[c];
+    per = 3; io = c ? 2 : 0; ix = 1;
   }
+#pragma unroll
+  for (int m = 0; m < kE; ++m) {
+    const int64_t idx = ed_index(d, a0 + s0 * m, a1 + s1 * m) * per;
+    ev.own[m] = ldc<R>(d.tab, idx + io);
+    if (M > 1 && d.form >= 3) ev.other[m] = ldc<R>(d.tab, idx + ix);
+  }
+}
+
+// Apply row c of exp(-i dt D) / prod(N).  For the 2x2 forms the partner
+// component's values at the same k points come through LDS: `mine` and
+// `theirs` are this thread's and the partner's padded line slots, and the
+// exchange is bracketed by workgroup barriers (uniform: the form is).
+template <typename R, int M>
+__device__ __forceinline__ void apply_ed(Cx<R> (&v)[kE], const EdDesc& d, const EdVals<R>& ev, int t, int T,
+                                         Cx<R>* mine, const Cx<R>* theirs) {
+  if (d.form == 0) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) v[m] = scale(v[m], (R)d.scale);
+    return;
+  }
+  if (M > 1 && d.form >= 3) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kE; ++m) mine[lds_pad(t + m * T)] = v[m];
+    __syncthreads();
+    Cx<R> p[kE];
+#pragma unroll
+    for (int m = 0; m < kE; ++m) p[m] = theirs[lds_pad(t + m * T)];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kE; ++m) v[m] = ev.own[m] * v[m] + ev.other[m] * p[m];
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < kE; ++m) v[m] = ev.own[m] * v[m];
+}
+
+template <typename R>
+__device__ __forceinline__ Cx<R> (&as_bundle(Cx<R> (&v)[kE]))[1][kE] {
+  return reinterpret_cast<Cx<R>(&)[1][kE]>(v);
 }
 
 // ------------------------------------------------------------------ x-pass
@@ -203,51 +306,54 @@ struct XPassArgs {
   PointTerms pt;
 };
 
+// LDS of the x-pass / 1D kernel: two FFT buffers of M*W padded lines.  The
+// |u_j|^2 exchange of coupled nonlinearities reuses the first buffer (it runs
+// between transforms, fenced by barriers).
 template <typename R, int N, int M, int W>
-__global__ void __launch_bounds__(W * (N / kE)) x_pass_kernel(XPassArgs a) {
+__host__ __device__ constexpr size_t xlds_bytes() {
+  return 2ull * M * W * lds_line_stride(N) * sizeof(Cx<R>);
+}
+
+template <typename R, int N, int M, int W, bool NOISE>
+__global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
   constexpr int T = N / kE;
   constexpr int LS = lds_line_stride(N);
   extern __shared__ unsigned char smem_raw[];
   Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
   const int tid = threadIdx.x;
-  const int l = tid / T, t = tid % T;
+  const int c = tid / (W * T);
+  const int l = (tid / T) % W, t = tid % T;
   const int64_t line_raw = (int64_t)blockIdx.x * W + l;
   const bool valid = line_raw < a.nlines;
   const int64_t line = valid ? line_raw : a.nlines - 1;
   const int64_t y = line % a.ny;
-  const int64_t sp_off = y * N;
-  Cx<R> v[M][kE];
+  Cx<R>* uc = reinterpret_cast<Cx<R>*>(c ? a.u[M - 1] : a.u[0]) + line * N;
+  Cx<R> v[kE];
 #pragma unroll
-  for (int c = 0; c < M; ++c) {
-    const Cx<R>* src = reinterpret_cast<const Cx<R>*>(a.u[c]) + line * N;
-#pragma unroll
-    for (int m = 0; m < kE; ++m) v[c][m] = src[t + m * T];
-  }
-  Cx<R>* my = lds + l * LS;
-  const int cstride = W * LS, boff = M * W * LS;
-  const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(a.tw);
+  for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
+  TwReg<R, N> tr;
+  if (a.flags & (kInK | kOutK)) load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
+  PointCache<R, M> pc;
+  if (a.flags & (kEnd | kStart)) load_point_cache<R, M>(pc, a.pt, c, t, T, y, y * N);
+  Cx<R>* my = lds + (c * W + l) * LS;
+  const int boff = M * W * LS;
+  R* a2 = reinterpret_cast<R*>(lds) + (size_t)l * M * N;
   if (a.flags & kInK) {
-    fft_lines<R, N, T, M, true>(v, my, cstride, boff, t, tw, a.tws);
+    fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
     __syncthreads();
   }
-  if (a.flags & kEnd) half_step<R, N, M>(v, a.h_end, a.pt, t, T, line, y, sp_off);
-  if (a.frame[0] != nullptr && valid) {
+  if (a.flags & kEnd) half_step<R, N, M, NOISE>(v, c, a.h_end, a.pt, pc, t, T, line, a2);
+  void* fr = c ? a.frame[M - 1] : a.frame[0];
+  if (fr != nullptr && valid) {
+    Cx<R>* dst = reinterpret_cast<Cx<R>*>(fr) + line * N;
 #pragma unroll
-    for (int c = 0; c < M; ++c) {
-      Cx<R>* dst = reinterpret_cast<Cx<R>*>(a.frame[c]) + line * N;
-#pragma unroll
-      for (int m = 0; m < kE; ++m) dst[t + m * T] = v[c][m];
-    }
+    for (int m = 0; m < kE; ++m) dst[t + m * T] = v[m];
   }
-  if (a.flags & kStart) half_step<R, N, M>(v, a.h_start, a.pt, t, T, line, y, sp_off);
-  if (a.flags & kOutK) fft_lines<R, N, T, M, false>(v, my, cstride, boff, t, tw, a.tws);
+  if (a.flags & kStart) half_step<R, N, M, NOISE>(v, c, a.h_start, a.pt, pc, t, T, line, a2);
+  if (a.flags & kOutK) fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
   if (valid) {
 #pragma unroll
-    for (int c = 0; c < M; ++c) {
-      Cx<R>* dst = reinterpret_cast<Cx<R>*>(a.u[c]) + line * N;
-#pragma unroll
-      for (int m = 0; m < kE; ++m) dst[t + m * T] = v[c][m];
-    }
+    for (int m = 0; m < kE; ++m) uc[t + m * T] = v[m];
   }
 }
 
@@ -257,63 +363,55 @@ struct YPassArgs {
   int64_t nx;         // dim-0 extent (columns)
   int64_t ncolblk;    // nx / C
   int64_t nblocks;    // ncolblk * batch
-  int xcd_remap;      // 1: spread consecutive column blocks over one XCD
-  int ed_form;
-  const void* ed;     // exp(-i dt D) / (nx ny), nx*ny entries of 1, M or M*M values
-  double scale;       // 1/(nx ny) when ed_form == 0
+  int xcd_remap;      // 1: give each XCD a contiguous range of column blocks
+  EdDesc ed;
   const void* tw;
   int tws;
 };
 
 template <typename R, int N, int M, int C>
-__global__ void __launch_bounds__(C * (N / kE)) y_pass_kernel(YPassArgs a) {
+__host__ __device__ constexpr size_t ylds_bytes() {
+  return 2ull * M * C * lds_line_stride(N) * sizeof(Cx<R>);
+}
+
+template <typename R, int N, int M, int C>
+__global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
   constexpr int T = N / kE;
   constexpr int LS = lds_line_stride(N);
   extern __shared__ unsigned char smem_raw[];
   Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
   int64_t blk = blockIdx.x;
   if (a.xcd_remap) {
-    // Workgroups are dispatched round-robin over 8 XCDs; give each XCD a
-    // contiguous range of column blocks so neighbouring columns that share
-    // cache lines meet in one L2.
+    // Workgroups are dispatched round-robin over the 8 XCDs; map them so each
+    // XCD works on a contiguous range of column blocks and neighbouring
+    // columns that share cache lines meet in one L2.
     const int64_t per = a.nblocks / 8;
     blk = (blk % 8) * per + blk / 8;
   }
   const int64_t b = blk / a.ncolblk;
   const int64_t x0 = (blk % a.ncolblk) * C;
   const int tid = threadIdx.x;
-  const int cx = tid % C, t = tid / C;
+  const int c = tid / (C * T);
+  const int r = tid % (C * T);
+  const int cx = r % C, t = r / C;
   const int64_t nx = a.nx;
-  const int64_t plane = nx * N;
-  const int64_t off = b * plane + x0 + cx;
-  Cx<R> v[M][kE];
+  Cx<R>* uc = reinterpret_cast<Cx<R>*>(c ? a.u[M - 1] : a.u[0]) + b * nx * N + x0 + cx;
+  Cx<R> v[kE];
 #pragma unroll
-  for (int c = 0; c < M; ++c) {
-    const Cx<R>* src = reinterpret_cast<const Cx<R>*>(a.u[c]) + off;
-#pragma unroll
-    for (int m = 0; m < kE; ++m) v[c][m] = src[(int64_t)(t + m * T) * nx];
-  }
-  Cx<R>* my = lds + cx * LS;
-  const int cstride = C * LS, boff = M * C * LS;
-  const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(a.tw);
-  fft_lines<R, N, T, M, false>(v, my, cstride, boff, t, tw, a.tws);
-#pragma unroll
-  for (int m = 0; m < kE; ++m) {
-    Cx<R> w[M];
-#pragma unroll
-    for (int c = 0; c < M; ++c) w[c] = v[c][m];
-    apply_ed<R, M>(w, a.ed_form, a.ed, (int64_t)(t + m * T) * nx + x0 + cx, (R)a.scale);
-#pragma unroll
-    for (int c = 0; c < M; ++c) v[c][m] = w[c];
-  }
+  for (int m = 0; m < kE; ++m) v[m] = uc[(int64_t)(t + m * T) * nx];
+  TwReg<R, N> tr;
+  load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
+  EdVals<R> ev;
+  load_ed<R, M>(ev, a.ed, c, x0 + cx, 0, t, T);
+  Cx<R>* my = lds + (c * C + cx) * LS;
+  const Cx<R>* partner = lds + ((M - 1 - c) * C + cx) * LS;
+  const int boff = M * C * LS;
+  fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
+  apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
   __syncthreads();
-  fft_lines<R, N, T, M, true>(v, my, cstride, boff, t, tw, a.tws);
+  fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
 #pragma unroll
-  for (int c = 0; c < M; ++c) {
-    Cx<R>* dst = reinterpret_cast<Cx<R>*>(a.u[c]) + off;
-#pragma unroll
-    for (int m = 0; m < kE; ++m) dst[(int64_t)(t + m * T) * nx] = v[c][m];
-  }
+  for (int m = 0; m < kE; ++m) uc[(int64_t)(t + m * T) * nx] = v[m];
 }
 
 // ------------------------------------------------------------ 1D whole step
@@ -326,35 +424,38 @@ struct Step1DArgs {
   int64_t env_base;     // sample index of F_now for the first half-step
   double delta;         // dt/2
   uint64_t call0;       // Philox counter of the first half-step
-  int ed_form;
-  const void* ed;
-  double scale;
+  EdDesc ed;
   const void* tw;
   int tws;
   PointTerms pt;
 };
 
-template <typename R, int N, int M, int W>
-__global__ void __launch_bounds__(W * (N / kE)) step1d_kernel(Step1DArgs a) {
+template <typename R, int N, int M, int W, bool NOISE>
+__global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) {
   constexpr int T = N / kE;
   constexpr int LS = lds_line_stride(N);
   extern __shared__ unsigned char smem_raw[];
   Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
   const int tid = threadIdx.x;
-  const int l = tid / T, t = tid % T;
+  const int c = tid / (W * T);
+  const int l = (tid / T) % W, t = tid % T;
   const int64_t line_raw = (int64_t)blockIdx.x * W + l;
   const bool valid = line_raw < a.nlines;
   const int64_t line = valid ? line_raw : a.nlines - 1;
-  Cx<R> v[M][kE];
+  Cx<R>* uc = reinterpret_cast<Cx<R>*>(c ? a.u[M - 1] : a.u[0]) + line * N;
+  Cx<R> v[kE];
 #pragma unroll
-  for (int c = 0; c < M; ++c) {
-    const Cx<R>* src = reinterpret_cast<const Cx<R>*>(a.u[c]) + line * N;
-#pragma unroll
-    for (int m = 0; m < kE; ++m) v[c][m] = src[t + m * T];
-  }
-  Cx<R>* my = lds + l * LS;
-  const int cstride = W * LS, boff = M * W * LS;
-  const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(a.tw);
+  for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
+  TwReg<R, N> tr;
+  load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
+  PointCache<R, M> pc;
+  load_point_cache<R, M>(pc, a.pt, c, t, T, 0, 0);
+  EdVals<R> ev;
+  load_ed<R, M>(ev, a.ed, c, t, T, 0, 0);
+  Cx<R>* my = lds + (c * W + l) * LS;
+  const Cx<R>* partner = lds + ((M - 1 - c) * W + l) * LS;
+  const int boff = M * W * LS;
+  R* a2 = reinterpret_cast<R*>(lds) + (size_t)l * M * N;
   const double q = 0.5 * a.delta;  // (dt/2)/2 multiplies F
   for (int64_t s = 0; s < a.nsteps; ++s) {
     HalfStep h;
@@ -373,38 +474,27 @@ __global__ void __launch_bounds__(W * (N / kE)) step1d_kernel(Step1DArgs a) {
     h.fnext[0] = q * en[1][0];
     h.fnext[1] = q * en[1][1];
     h.call = a.call0 + 2 * (uint64_t)s;
-    half_step<R, N, M>(v, h, a.pt, t, T, line, 0, 0);
-    fft_lines<R, N, T, M, false>(v, my, cstride, boff, t, tw, a.tws);
-#pragma unroll
-    for (int m = 0; m < kE; ++m) {
-      Cx<R> w[M];
-#pragma unroll
-      for (int c = 0; c < M; ++c) w[c] = v[c][m];
-      apply_ed<R, M>(w, a.ed_form, a.ed, t + m * T, (R)a.scale);
-#pragma unroll
-      for (int c = 0; c < M; ++c) v[c][m] = w[c];
-    }
+    half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line, a2);
+    fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
+    apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
     __syncthreads();
-    fft_lines<R, N, T, M, true>(v, my, cstride, boff, t, tw, a.tws);
+    fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
     h.fnow[0] = q * en[1][0];
     h.fnow[1] = q * en[1][1];
     h.fnext[0] = q * en[2][0];
     h.fnext[1] = q * en[2][1];
     h.call += 1;
-    half_step<R, N, M>(v, h, a.pt, t, T, line, 0, 0);
+    half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line, a2);
     __syncthreads();
   }
   if (valid) {
 #pragma unroll
-    for (int c = 0; c < M; ++c) {
-      Cx<R>* dst = reinterpret_cast<Cx<R>*>(a.u[c]) + line * N;
+    for (int m = 0; m < kE; ++m) uc[t + m * T] = v[m];
+    void* fr = c ? a.frame[M - 1] : a.frame[0];
+    if (fr != nullptr) {
+      Cx<R>* dst = reinterpret_cast<Cx<R>*>(fr) + line * N;
 #pragma unroll
-      for (int m = 0; m < kE; ++m) dst[t + m * T] = v[c][m];
-      if (a.frame[0] != nullptr) {
-        Cx<R>* fr = reinterpret_cast<Cx<R>*>(a.frame[c]) + line * N;
-#pragma unroll
-        for (int m = 0; m < kE; ++m) fr[t + m * T] = v[c][m];
-      }
+      for (int m = 0; m < kE; ++m) dst[t + m * T] = v[m];
     }
   }
 }
@@ -419,7 +509,7 @@ struct EdArgs {
   const double* D;  // complex FP64 table, npts * per entries
   void* out;
   int64_t npts;
-  int form;         // 1 scalar, 2 diag, 3 full
+  int form;         // 1 scalar, 2 diag, 3 full 2x2, 4 symmetric 2x2 (a, b, d)
   int M;
   double dt;
   double scale;
@@ -493,10 +583,18 @@ __global__ void ed_table_kernel(EdArgs a) {
       const Cx<double> e = cexp_d(mk<double>(a.dt * z.im, -a.dt * z.re));
       out[k * per + i] = mk<R>((R)(e.re * a.scale), (R)(e.im * a.scale));
     }
-  } else if (a.M == 1) {
-    const Cx<double> z = D[k];
-    const Cx<double> e = cexp_d(mk<double>(a.dt * z.im, -a.dt * z.re));
-    out[k] = mk<R>((R)(e.re * a.scale), (R)(e.im * a.scale));
+  } else if (a.form == 4) {
+    // symmetric 2x2 stored as (a, b, d); exp of a symmetric matrix is symmetric
+    Cx<double> A[4], E[4];
+    const int src[4] = {0, 1, 1, 2};
+    for (int i = 0; i < 4; ++i) {
+      const Cx<double> z = D[k * 3 + src[i]];
+      A[i] = mk<double>(a.dt * z.im, -a.dt * z.re);
+    }
+    expm2x2_d(A, E);
+    const int dst[3] = {0, 1, 3};
+    for (int i = 0; i < 3; ++i)
+      out[k * 3 + i] = mk<R>((R)(E[dst[i]].re * a.scale), (R)(E[dst[i]].im * a.scale));
   } else {
     Cx<double> A[4], E[4];
     for (int i = 0; i < 4; ++i) {

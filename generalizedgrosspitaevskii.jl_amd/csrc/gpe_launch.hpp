@@ -5,12 +5,14 @@
 
 namespace gpe {
 
-// Lines per workgroup of the x-pass and of the 1D whole-step kernel.
-__host__ __device__ constexpr int x_lines_per_wg(int N) { return N >= 1024 ? 1 : 1024 / N; }
+// Lines per workgroup of the x-pass and of the 1D whole-step kernel: W*N/8
+// threads per component, at least one wavefront, 64..512 threads.
+__host__ __device__ constexpr int x_lines_per_wg(int N) { return N >= 512 ? 1 : 512 / N; }
 
-// Columns per workgroup of the y-pass: about 256 threads, at most 16 columns.
+// Columns per workgroup of the y-pass: C*N/8 threads per component, at least
+// one wavefront, at most 16 columns.
 __host__ __device__ constexpr int y_cols_per_wg(int N) {
-  return (256 / (N / kE)) < 1 ? 1 : ((256 / (N / kE)) > 16 ? 16 : 256 / (N / kE));
+  return (128 / (N / kE)) < 1 ? 1 : ((128 / (N / kE)) > 16 ? 16 : 128 / (N / kE));
 }
 
 }  // namespace gpe

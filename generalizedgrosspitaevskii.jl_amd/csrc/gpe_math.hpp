@@ -1,0 +1,91 @@
+// Compact sine/cosine for the real-space half-step.
+//
+// Method: Cody-Waite reduction x = k pi/2 + r with k = rint(2x/pi) and a
+// three-term split of pi/2 applied with fused multiply-adds, then the Taylor
+// series of sin and cos on |r| <= pi/4 (to r^17 / r^18 in FP64, r^11 / r^12
+// in FP32: truncation below 1e-19 and 2e-9 relative), and the quadrant
+// rotation by k mod 4.  Arguments outside the range where the split is exact
+// (|x| >= 1e9 in FP64, 1e5 in FP32) and non-finite ones take the library
+// routine out of line, so the inlined code stays a few dozen instructions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gpe {
+
+// The out-of-line routine returns the pair by value: results passed through
+// pointers would live in scratch memory and be reloaded on every call site's
+// fast path too.
+template <typename R>
+struct SinCos {
+  R s, c;
+};
+__host__ __device__ __noinline__ inline SinCos<double> sincos_wide(double x) { return {sin(x), cos(x)}; }
+__host__ __device__ __noinline__ inline SinCos<float> sincos_wide(float x) { return {sinf(x), cosf(x)}; }
+
+__host__ __device__ __forceinline__ void fast_sincos(double x, double& s, double& c) {
+  if (!(fabs(x) < 1e9)) {
+    const SinCos<double> w = sincos_wide(x);
+    s = w.s;
+    c = w.c;
+    return;
+  }
+  const double k = rint(x * 0.6366197723675814);
+  double r = fma(-k, 1.5707963267948966, x);
+  r = fma(-k, 6.123233995736766e-17, r);
+  r = fma(-k, -1.4973849048591698e-33, r);
+  const double r2 = r * r;
+  double p = 2.8114572543455206e-15;        //  1/17!
+  p = fma(p, r2, -7.647163731819816e-13);   // -1/15!
+  p = fma(p, r2, 1.6059043836821613e-10);   //  1/13!
+  p = fma(p, r2, -2.505210838544172e-08);   // -1/11!
+  p = fma(p, r2, 2.7557319223985893e-06);   //  1/9!
+  p = fma(p, r2, -0.0001984126984126984);   // -1/7!
+  p = fma(p, r2, 0.008333333333333333);     //  1/5!
+  p = fma(p, r2, -0.16666666666666666);     // -1/3!
+  const double sr = fma(p * r2, r, r);
+  double q = -1.5619206968586225e-16;       // -1/18!
+  q = fma(q, r2, 4.779477332387385e-14);    //  1/16!
+  q = fma(q, r2, -1.1470745597729725e-11);  // -1/14!
+  q = fma(q, r2, 2.08767569878681e-09);     //  1/12!
+  q = fma(q, r2, -2.755731922398589e-07);   // -1/10!
+  q = fma(q, r2, 2.48015873015873e-05);     //  1/8!
+  q = fma(q, r2, -0.001388888888888889);    // -1/6!
+  q = fma(q, r2, 0.041666666666666664);     //  1/4!
+  q = fma(q, r2, -0.5);                     // -1/2!
+  const double cr = fma(q, r2, 1.0);
+  const int quad = (int)((long long)k & 3);
+  s = (quad == 0) ? sr : (quad == 1) ? cr : (quad == 2) ? -sr : -cr;
+  c = (quad == 0) ? cr : (quad == 1) ? -sr : (quad == 2) ? -cr : sr;
+}
+
+__host__ __device__ __forceinline__ void fast_sincos(float x, float& s, float& c) {
+  if (!(fabsf(x) < 1e5f)) {
+    const SinCos<float> w = sincos_wide(x);
+    s = w.s;
+    c = w.c;
+    return;
+  }
+  const float k = rintf(x * 0.63661975f);
+  float r = fmaf(-k, 1.5707963705062866f, x);
+  r = fmaf(-k, -4.371138828673793e-08f, r);
+  r = fmaf(-k, -1.7151245100058819e-15f, r);
+  const float r2 = r * r;
+  float p = -2.5052107943679403e-08f;       // -1/11!
+  p = fmaf(p, r2, 2.7557318844628753e-06f); //  1/9!
+  p = fmaf(p, r2, -0.00019841270113829523f);// -1/7!
+  p = fmaf(p, r2, 0.008333333767950535f);   //  1/5!
+  p = fmaf(p, r2, -0.1666666716337204f);    // -1/3!
+  const float sr = fmaf(p * r2, r, r);
+  float q = 2.0876755879584152e-09f;        //  1/12!
+  q = fmaf(q, r2, -2.755731998149713e-07f); // -1/10!
+  q = fmaf(q, r2, 2.4801587642286904e-05f); //  1/8!
+  q = fmaf(q, r2, -0.0013888889225199819f); // -1/6!
+  q = fmaf(q, r2, 0.0416666679084301f);     //  1/4!
+  q = fmaf(q, r2, -0.5f);                   // -1/2!
+  const float cr = fmaf(q, r2, 1.0f);
+  const int quad = (int)k & 3;
+  s = (quad == 0) ? sr : (quad == 1) ? cr : (quad == 2) ? -sr : -cr;
+  c = (quad == 0) ? cr : (quad == 1) ? -sr : (quad == 2) ? -cr : sr;
+}
+
+}  // namespace gpe
