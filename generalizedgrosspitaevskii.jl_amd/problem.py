@@ -425,6 +425,9 @@ class StrangSplittingIterator:
 
     def __init__(self, prob, tspan, dt, nsaves, save_start=True, rng=None):
         lib = L.load()
+        if prob.noise_prototype is not None and prob.position_noise_func is None:
+            # _mul(AdditiveIdentity, SVector) has no method in the reference (src/kernels.jl:9,42)
+            raise TypeError("noise_prototype without position_noise_func (no method in the reference)")
         self.prob = prob
         self.tspan = (float(tspan[0]), float(tspan[1]))
         self.nsaves = int(nsaves)
@@ -497,9 +500,6 @@ class StrangSplittingIterator:
                 L.check(lib.gpe_set_pump(h, kind, prof.ctypes.data, None if env is None else env.ctypes.data,
                                          0 if env is None else len(env) // 2))
         if prob.noise_prototype is not None:
-            if prob.position_noise_func is None:
-                # _mul(AdditiveIdentity, SVector) has no method in the reference (src/kernels.jl:9,42)
-                raise TypeError("noise_prototype without position_noise_func (no method in the reference)")
             proto = prob.noise_prototype
             first = proto[0] if isinstance(proto, (tuple, list)) else proto
             cplx = np.iscomplexobj(np.empty(0, dtype=str(first.dtype).replace("torch.", "")))

@@ -1,0 +1,91 @@
+"""Host-side mirror of the reference API: closure -> descriptor reduction
+and the reference's argument checks (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+
+def test_problem_checks_like_reference(ggp):
+    # src/problem.jl:112-113
+    with pytest.raises(AssertionError):
+        ggp.GrossPitaevskiiProblem((np.zeros(8),), (1.0, 1.0))
+    with pytest.raises(AssertionError):
+        ggp.GrossPitaevskiiProblem((np.zeros((8, 8)), np.zeros((8, 4))), (1.0, 1.0))
+    p = ggp.GrossPitaevskiiProblem((np.zeros((8, 8)),), (2, 3))
+    assert p.u0[0].dtype == np.complex128 and p.lengths == (2.0, 3.0) and repr(p) == "2D GrossPitaevskiiProblem"
+    p32 = ggp.GrossPitaevskiiProblem((np.zeros(8, np.float32),), (1,))
+    assert p32.u0[0].dtype == np.complex64
+
+
+def test_grids_match_reference_definitions(ggp):
+    p = ggp.GrossPitaevskiiProblem((np.zeros((16, 8)),), (4.0, 2.0))
+    x, y = ggp.direct_grid(p)
+    assert np.array_equal(x, np.arange(16) * 0.25) and np.array_equal(y, np.arange(8) * 0.25)
+    kx, ky = ggp.reciprocal_grid(p)
+    assert np.array_equal(kx, np.r_[np.arange(8), np.arange(-8, 0)] * (2 * math.pi / 4.0))
+
+
+def test_nonlinearity_fit(ggp):
+    from gpe_hip_amd import problem as P
+
+    g, c = P.fit_nonlinearity(lambda u, p: [0 * u[0], p * abs(u[1]) ** 2], 2, 0.015)
+    assert np.allclose(g, [[0, 0], [0, 0.015]], atol=1e-17) and np.all(c == 0)
+    g, c = P.fit_nonlinearity(lambda u, p: [[abs(u[0]) ** 2 - 0.5j, 0], [0, 2 * abs(u[0]) ** 2 + abs(u[1]) ** 2]],
+                              2, None)
+    assert np.allclose(g, [[1, 0], [2, 1]]) and np.allclose(c, [-0.5j, 0])
+    g, c = P.fit_nonlinearity(lambda u, p: -6 * abs(u[0]) ** 2 / 2, 1, None)
+    assert np.allclose(g, [[-3]]) and np.allclose(c, [0])
+    with pytest.raises(NotImplementedError):
+        P.fit_nonlinearity(lambda u, p: abs(u[0]) ** 4, 1, None)
+    with pytest.raises(NotImplementedError):
+        P.fit_nonlinearity(lambda u, p: [[0 * u[0], 1 + 0 * u[0]], [1 + 0 * u[0], 0 * u[0]]], 2, None)
+
+
+def test_noise_eta(ggp):
+    from gpe_hip_amd import problem as P
+
+    axes = (np.arange(8) * 1.0,)
+    assert np.allclose(P.noise_eta(lambda u, r, p: 0.25, 1, None, axes), [0.25])
+    assert np.allclose(P.noise_eta(lambda u, r, p: [0.1, 0.2], 2, None, axes), [0.1, 0.2])
+    with pytest.raises(NotImplementedError):
+        P.noise_eta(lambda u, r, p: 0.1 * abs(u[0]), 1, None, axes)
+    with pytest.raises(NotImplementedError):
+        P.noise_eta(lambda u, r, p: 0.1j, 1, None, axes)
+
+
+def test_pump_descriptor_separable_and_static(ggp):
+    from gpe_hip_amd import _lib as Lb
+    from gpe_hip_amd import problem as P
+
+    N, L = 32, 8.0
+    p = ggp.GrossPitaevskiiProblem((np.zeros((N, N)),) * 2, (L, L),
+                                   pump=lambda r, q, t: [np.exp(-((r[0] - 4) ** 2 + (r[1] - 4) ** 2)), 0 * r[0]])
+    rax = P._bcast_axes(ggp.direct_grid(p))
+    kind, prof, env = P.pump_descriptor(p, rax, (N, N), np.linspace(0, 1, 21))
+    assert kind == Lb.GPE_PUMP_SEPARABLE and env is None and prof.size == 2 * 2 * (N + N)
+    # a non-separable profile falls back to the full table
+    p2 = ggp.GrossPitaevskiiProblem((np.zeros((N, N)),), (L, L),
+                                    pump=lambda r, q, t: np.exp(-((r[0] - 4) ** 2) * (1 + (r[1] - 4) ** 2)))
+    kind, prof, env = P.pump_descriptor(p2, rax, (N, N), np.linspace(0, 1, 21))
+    assert kind == Lb.GPE_PUMP_TABLE and prof.size == 2 * N * N
+    # time dependence F = P(r) e(t) gives one envelope sample per pump time
+    times = np.linspace(0, 2, 41)
+    p3 = ggp.GrossPitaevskiiProblem((np.zeros(N),), (L,), pump=lambda r, q, t: np.exp(-r[0] ** 2) * math.sqrt(t))
+    kind, prof, env = P.pump_descriptor(p3, P._bcast_axes(ggp.direct_grid(p3)), (N,), times)
+    e = env.view(np.complex128)
+    # profile taken at the probe time of largest norm (t = 2): F = P(r) e(t), e(2) = 1
+    assert e.shape == (41,) and np.allclose(e, np.sqrt(times / 2.0))
+    # not separable in time
+    p4 = ggp.GrossPitaevskiiProblem((np.zeros(N),), (L,), pump=lambda r, q, t: np.exp(-(r[0] - t) ** 2))
+    with pytest.raises(NotImplementedError):
+        P.pump_descriptor(p4, P._bcast_axes(ggp.direct_grid(p4)), (N,), times)
+
+
+def test_noise_prototype_without_noise_func_is_an_error(ggp):
+    """_mul(AdditiveIdentity, SVector) has no method in the reference (Q9)."""
+    prob = ggp.GrossPitaevskiiProblem((np.zeros(16, complex),), (1.0,), noise_prototype=(np.zeros(16, complex),))
+    with pytest.raises(TypeError):
+        ggp.init(prob, ggp.StrangSplitting(), (0, 1), dt=0.1, nsaves=1)
+    with pytest.raises(TypeError):
+        ggp.init(prob, object(), (0, 1), dt=0.1, nsaves=1)

@@ -1,0 +1,323 @@
+"""Pins of the FP64 oracle (oracle/gpe_oracle.py) that do not retype its
+formulas: known answers, closed-form solutions, invariants, an independent
+library (scipy), and the reference's own analytic tests.
+
+Reference tests restated here (expected values computed at test time in the
+reference too; it stores no golden vectors):
+    test/windowed_ft.jl          vacuum commutator of windowed FTs, rtol 7e-2
+    test/exciton_polariton_test.jl  homogeneous steady-state relations, 3e-2
+Closed forms replace StructuredLight's free_propagation / kerr_propagation
+(test/free_propagation.jl:16, test/kerr_propagation.jl:17), which are not in
+this image.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import gpe_oracle as orc
+
+
+# ------------------------------------------------ src/fixed_time_stepping.jl
+@pytest.mark.parametrize("dt,tspan,nsaves,expect_dt,expect_sps", [
+    (0.01, (0, 0.4), 64, 0.00625, 1),                     # README Kerr example
+    (0.05, (0, 3300), 512, 0.04996366279069767, 129),     # test/bistability_cycle.jl
+    (0.1, (0, 100), 256, 0.09765625, 4),                  # test/exciton_polariton_test.jl
+    (4, (0, 200), 1, 4.0, 50),                            # test/windowed_ft.jl
+    (0.05, (0, 500), 100, 0.05, 100),                     # 512^2 polariton workload
+])
+def test_resolve_fixed_timestepping_known_cases(dt, tspan, nsaves, expect_dt, expect_sps):
+    got_dt, sps, t0 = orc.resolve_fixed_timestepping(dt, tspan, nsaves)
+    assert got_dt == expect_dt and sps == expect_sps and t0 == tspan[0]
+
+
+def test_time_accumulation_is_repeated_addition():
+    # t += dt (fixed_time_stepping.jl:44): 10^4 steps of 0.05 do not land on 500
+    p = orc.Problem((np.zeros(16, complex),), (1.0,))
+    ts, _, _ = orc.solve(p, (0, 500), 0.05, 100)
+    assert ts[-1] == 500.0000000000794
+    assert ts[1] == sum([0.05] * 100)
+
+
+# ----------------------------------------------------------- src/problem.jl
+def test_grids():
+    x = orc.direct_grid_1d(8.0, 128)
+    assert x[0] == 0.0 and x[-1] == 127 * 0.0625 and np.all(np.diff(x) == 0.0625)
+    k = orc.reciprocal_grid_1d(8.0, 128)
+    # FFT order [0..N/2-1, -N/2..-1] * 2pi/L, bit-exact for power-of-two N
+    m = np.r_[np.arange(64), np.arange(-64, 0)]
+    assert np.array_equal(k, m * (2 * math.pi / 8.0))
+    np.testing.assert_allclose(k, 2 * math.pi * np.fft.fftfreq(128, 8.0 / 128), rtol=1e-15)
+
+
+# ------------------------------------------------------------------ Philox
+def test_philox_known_answers():
+    """Random123 known-answer vectors of Philox4x32-10."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in kat:
+        got = tuple(int(x) for x in orc.philox4x32_10(*ctr, *key))
+        assert got == want
+
+
+def test_xi_statistics():
+    """E xi = 0, E|xi|^2 = 1 with Re, Im ~ N(0, 1/2) (src/problem.jl:38-41);
+    real xi ~ N(0, 1); draws of different calls/components uncorrelated."""
+    shape = (256, 512)
+    z = orc.philox_xi(shape, 0, 1234, 7)
+    n = z.size
+    tol = 5 / math.sqrt(n)
+    assert abs(z.mean()) < tol
+    assert abs(np.mean(abs(z) ** 2) - 1) < 5 * tol
+    assert abs(np.var(z.real) - 0.5) < 5 * tol and abs(np.var(z.imag) - 0.5) < 5 * tol
+    assert abs(np.mean(z.real * z.imag)) < tol
+    z2 = orc.philox_xi(shape, 0, 1234, 8)
+    z3 = orc.philox_xi(shape, 1, 1234, 7)
+    assert abs(np.mean(z * z2.conj())) < tol and abs(np.mean(z * z3.conj())) < tol
+    r = orc.philox_xi(shape, 0, 1234, 7, complex_noise=False)
+    assert np.all(r.imag == 0) and abs(np.var(r.real) - 1) < 10 * tol
+    # Gaussian tails: P(|Re xi| > 3 sqrt(1/2)) = 0.0027
+    frac = np.mean(abs(z.real) > 3 * math.sqrt(0.5))
+    assert abs(frac - 0.0027) < 0.0006
+
+
+# ------------------------------------------------------- matrix exponential
+def test_expm_2x2_against_scipy():
+    rng = np.random.default_rng(0)
+    for scale in (1e-6, 1e-3, 0.1, 1.0, 5.0):
+        A = (rng.standard_normal((2, 2, 50)) + 1j * rng.standard_normal((2, 2, 50))) * scale
+        E = orc.expm_2x2(A)
+        for i in range(50):
+            ref = scipy.linalg.expm(A[:, :, i])
+            assert np.max(abs(E[:, :, i] - ref)) <= 1e-14 * max(1.0, np.max(abs(ref)))
+
+
+def test_expm_2x2_degenerate_and_nilpotent():
+    # s -> 0: equal diagonal, zero coupling; nilpotent [[0,1],[0,0]]: exp = I + A
+    A = np.zeros((2, 2, 3), complex)
+    A[:, :, 0] = [[0.3 + 0.1j, 0], [0, 0.3 + 0.1j]]
+    A[:, :, 1] = [[0, 1], [0, 0]]
+    A[:, :, 2] = [[1e-9, 2e-9j], [3e-9, -1e-9]]
+    E = orc.expm_2x2(A)
+    np.testing.assert_allclose(E[:, :, 0], np.exp(0.3 + 0.1j) * np.eye(2), rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(E[:, :, 1], [[1, 1], [0, 1]], atol=1e-15)
+    np.testing.assert_allclose(E[:, :, 2], scipy.linalg.expm(A[:, :, 2]), atol=1e-17)
+
+
+# ------------------------------------------- closed-form evolutions (Strang)
+def _gauss2d(N, L, x0, y0, w, kx=0.0):
+    r = np.arange(N) * (L / N)
+    return np.exp(-((r[:, None] - x0) ** 2 + (r[None, :] - y0) ** 2) / w ** 2 + 1j * kx * r[:, None])
+
+
+def test_free_propagation_exact():
+    """D = |k|^2/2 only: Strang reduces to the exact spectral propagator,
+    u(k, t) = exp(-i |k|^2 t / 2) u(k, 0) (replaces StructuredLight's
+    free_propagation of test/free_propagation.jl:16)."""
+    N, L = 64, 10.0
+    u0 = _gauss2d(N, L, 5.0, 4.0, 1.3, kx=2.0)
+    ts, res, _ = orc.solve(orc.Problem((u0,), (L, L), dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2),
+                           (0, 2.0), 0.07, 20)
+    k = orc.reciprocal_grid_1d(L, N)
+    kk = k[:, None] ** 2 + k[None, :] ** 2
+    U0 = np.fft.fft2(u0)
+    for f, t in enumerate(ts):
+        exact = np.fft.ifft2(np.exp(-0.5j * kk * t) * U0)
+        assert np.linalg.norm(res[0][..., f] - exact) <= 1e-12 * np.linalg.norm(exact)
+
+
+def test_homogeneous_kerr_exact():
+    """Constant field, D(0) = 0, G = g|u|^2: u(t) = u0 exp(-i g |u0|^2 t)."""
+    a, g = 0.7 - 0.2j, -1.7
+    u0 = np.full((32, 32), a)
+    ts, res, _ = orc.solve(orc.Problem((u0,), (5.0, 5.0), dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2,
+                                       nonlinearity=lambda u, p: g * abs(u[0]) ** 2), (0, 3.0), 0.01, 6)
+    for f, t in enumerate(ts):
+        np.testing.assert_allclose(res[0][..., f], a * np.exp(-1j * g * abs(a) ** 2 * t), rtol=2e-13)
+
+
+def test_pump_trapezoid_with_reference_times():
+    """No D, no G: every half-step adds (dt/4)(F(tau_prev) + F(tau)) with the
+    pump times t+dt/2, t+dt taken AFTER t += dt (strang_splitting.jl:87,89,
+    fixed_time_stepping.jl:44-45) and F_now = F(t0) from init (:58)."""
+    N, L = 16, 4.0
+    env = lambda t: 1.0 + 0.3 * t + 0.05j * t * t
+    prof = np.exp(-((np.arange(N) * (L / N) - 2.0) ** 2))
+    pump = lambda r, p, t: np.exp(-((r[0] - 2.0) ** 2)) * env(t)
+    dt, nsaves, tspan = 0.1, 3, (0.0, 1.2)
+    ts, res, it = orc.solve(orc.Problem((np.zeros(N, complex),), (L,), pump=pump), tspan, dt, nsaves)
+    rdt, sps, _ = orc.resolve_fixed_timestepping(dt, tspan, nsaves)
+    taus = [0.0]
+    t = 0.0
+    expect = np.zeros(N, complex)
+    frames = []
+    for s in range(sps * nsaves):
+        t += rdt
+        for tau in (t + rdt / 2, t + rdt):
+            expect = expect + (rdt / 4) * (env(taus[-1]) + env(tau)) * prof
+            taus.append(tau)
+        if (s + 1) % sps == 0:
+            frames.append(expect.copy())
+    for f in range(nsaves):
+        np.testing.assert_allclose(res[0][..., f + 1], frames[f], rtol=1e-14, atol=1e-15)
+    # the lag matters: evaluating at the start-of-step times gives a visibly different answer
+    naive = np.zeros(N, complex)
+    for s in range(sps * nsaves):
+        t0 = s * rdt
+        naive = naive + (rdt / 2) * (env(t0) + env(t0 + rdt)) * prof
+    assert np.linalg.norm(naive - frames[-1]) > 1e-3 * np.linalg.norm(frames[-1])
+
+
+def test_norm_conserved_for_hermitian_problem():
+    """Real D(k), real G, no pump/noise: every factor is unitary."""
+    rng = np.random.default_rng(1)
+    u0 = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+    ts, res, _ = orc.solve(orc.Problem((u0,), (6.0, 6.0), dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2,
+                                       nonlinearity=lambda u, p: 0.8 * abs(u[0]) ** 2, potential=lambda r, p:
+                                       0.5 * (r[0] - 3) ** 2), (0, 2.0), 0.02, 4)
+    n0 = np.linalg.norm(u0)
+    for f in range(res[0].shape[-1]):
+        assert abs(np.linalg.norm(res[0][..., f]) / n0 - 1) < 1e-13
+
+
+def test_strang_is_second_order():
+    """Self-convergence on a focusing Kerr problem: error(dt)/error(dt/2) ~ 4
+    (replaces the comparison with StructuredLight's kerr_propagation)."""
+    N, L = 64, 12.0
+    u0 = _gauss2d(N, L, 6.0, 6.0, 1.5).astype(complex)
+    prob = lambda: orc.Problem((u0,), (L, L), dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2,
+                               nonlinearity=lambda u, p: -2.0 * abs(u[0]) ** 2)
+    out = {}
+    for dt in (0.04, 0.02, 0.0025):
+        _, res, _ = orc.solve(prob(), (0, 0.8), dt, 1)
+        out[dt] = res[0][..., -1]
+    e1 = np.linalg.norm(out[0.04] - out[0.0025])
+    e2 = np.linalg.norm(out[0.02] - out[0.0025])
+    assert 3.3 < e1 / e2 < 4.7
+
+
+def test_static_array_forms_agree():
+    """Scalar, SVector{1} and SMatrix{1,1} forms give the same solution
+    (test/bistability_cycle.jl:67-71, multicomponent_systems.md:59)."""
+    N, L = 32, 8.0
+    u0 = (_gauss2d(N, L, 4, 4, 1.0),)
+    d = lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2 - 0.1j
+    g = lambda u, p: 0.5 * abs(u[0]) ** 2
+    f = lambda r, p, t: 0.2 * np.exp(-((r[0] - 4) ** 2 + (r[1] - 4) ** 2)) * (1 + t)
+    forms = [(d, g, f),
+             (lambda k, p: [d(k, p)], lambda u, p: [g(u, p)], lambda r, p, t: [f(r, p, t)]),
+             (lambda k, p: [[d(k, p)]], lambda u, p: [[g(u, p)]], lambda r, p, t: [f(r, p, t)])]
+    sols = []
+    for dd, gg, ff in forms:
+        _, res, _ = orc.solve(orc.Problem(u0, (L, L), dispersion=dd, nonlinearity=gg, pump=ff), (0, 0.5), 0.05, 2)
+        sols.append(res[0])
+    for s in sols[1:]:
+        np.testing.assert_allclose(s, sols[0], rtol=1e-14, atol=1e-15)
+
+
+def test_batch_broadcast_equals_single_runs():
+    rng = np.random.default_rng(2)
+    B = 3
+    u0 = rng.standard_normal((32, B)) + 1j * rng.standard_normal((32, B))
+    kw = dict(dispersion=lambda k, p: k[0] ** 2 / 2 - 0.05j, nonlinearity=lambda u, p: abs(u[0]) ** 2,
+              pump=lambda r, p, t: 0.3 + 0 * r[0])
+    _, batched, _ = orc.solve(orc.Problem((u0,), (6.0,), **kw), (0, 1.0), 0.05, 2)
+    for b in range(B):
+        _, single, _ = orc.solve(orc.Problem((u0[:, b].copy(),), (6.0,), **kw), (0, 1.0), 0.05, 2)
+        assert np.array_equal(batched[0][:, b, :], single[0])
+
+
+# --------------------------------------------------------------- noise, TWA
+def test_noise_stationary_vacuum_occupation():
+    """d psi = -gamma/2 psi dt - i eta dW: stationary <|psi|^2> = eta^2/gamma."""
+    N, B, L = 32, 512, 10.0
+    gam, eta = 0.4, 0.3
+    prob = orc.Problem((np.zeros((N, B), complex),), (L,), dispersion=lambda k, p: -0.5j * gam + 0 * k[0],
+                       position_noise_func=lambda u, r, p: eta, noise_prototype="complex", noise_seed=11)
+    ts, res, _ = orc.solve(prob, (0, 60.0), 0.1, 1, save_start=False)
+    occ = np.mean(abs(res[0][..., -1]) ** 2)
+    expect = eta ** 2 / gam
+    # discrete-time correction of the exponential integrator is O(gamma dt)
+    assert abs(occ / expect - 1) < 0.03
+
+
+def test_reference_windowed_ft_commutator():
+    """test/windowed_ft.jl: vacuum fluctuations of a damped 1D field with
+    eta = sqrt(gamma / 2 dx); the windowed-FT correlation must match the
+    analytic commutator within rtol 7e-2 (same workload, Philox noise)."""
+    hb = 0.6582
+    gam = 0.1 / hb
+    m = hb ** 2 / 2.5
+    d0 = 0.49 / hb
+    L, N = 20.0, 64
+    dL = L / N
+    param = dict(d0=d0, m=m, gam=gam, hb=hb, dL=dL)
+
+    def dispersion(ks, p):
+        return -1j * p["gam"] / 2 + p["hb"] * ks[0] ** 2 / (2 * p["m"]) - p["d0"]
+
+    prob = orc.Problem((np.zeros((N, 10 ** 4), complex),), (L,), dispersion=dispersion, param=param,
+                       position_noise_func=lambda psi, r, p: math.sqrt(p["gam"] / 2 / p["dL"]),
+                       noise_prototype="complex", noise_seed=1234)
+    _, res, _ = orc.solve(prob, (0, 200), 4, 1, save_start=False)
+    sol = res[0][..., 0]
+    rs = -L / 2 + np.arange(N) * (L / N)
+    ks = -math.pi / L + np.arange(N) * (2 * math.pi / L)
+    window = lambda x, x0, w: np.exp(-(x - x0) ** 2 / w ** 2)
+
+    def correlation(w1, w2):
+        s1 = sol * w1[:, None]
+        s2 = sol * w2[:, None]
+        f1 = np.fft.ifftshift(np.fft.fft(np.fft.fftshift(s1, 0), axis=0), 0)
+        f2 = np.fft.ifftshift(np.fft.fft(np.fft.fftshift(s2, 0), axis=0), 0)
+        # g1[i, j] = dot(f2[j, :], f1[i, :]) / length(sol), dot conjugating its first argument
+        return f1 @ f2.conj().T / sol.size
+
+    def analytic(w1, w2):
+        out = np.empty((N, N), complex)
+        for i, k in enumerate(ks):
+            for j, kp in enumerate(ks):
+                out[i, j] = np.sum(np.exp(1j * (k - kp) * rs) * w1 * np.conj(w2))
+        return out / (rs[-1] - rs[0]) / 2
+
+    for x0 in (-1, 0, 1):
+        for w in (3, 4, 5):
+            w1, w2 = window(rs, x0, w), window(rs, -x0, w)
+            corr, an = correlation(w1, w2), analytic(w1, w2)
+            assert np.linalg.norm(corr - an) <= 7e-2 * max(np.linalg.norm(corr), np.linalg.norm(an))
+
+
+def test_reference_exciton_polariton_steady_state():
+    """test/exciton_polariton_test.jl: 2x2 SMatrix dispersion, SVector
+    nonlinearity and Gaussian photon pump; homogeneous steady-state relations
+    at the centre within 3e-2 (in the loss convention of the dispersion used,
+    examples/exciton_polariton.jl:117,121)."""
+    hb = 0.654
+    Wr = 5.07 / 2 / hb
+    gx = 0.0015 / hb
+    gc = 0.07 / 0.6571 / hb
+    wx, wc = 1484.44 / hb, 1482.76 / hb
+    m = hb ** 2 / (2 * 2e-1)
+    dx, dc = wc - wx, 0.0
+    A, w, g = 2.0, 100.0, 1e-2 / hb
+    L, N = 256.0, 128
+
+    def dispersion(k, p):
+        Dcc = hb * (k[0] ** 2 + k[1] ** 2) / (2 * m) - dc - 1j * gc
+        return [[Dcc, Wr], [Wr, -dx - 1j * gx]]
+
+    prob = orc.Problem((np.zeros((N, N), complex),) * 2, (L, L), dispersion=dispersion,
+                       nonlinearity=lambda u, p: [0 * u[0], g * abs(u[1]) ** 2],
+                       pump=lambda r, p, t: [A * np.exp(-((r[0] - L / 2) ** 2 + (r[1] - L / 2) ** 2) / w ** 2),
+                                             0 * r[0] * r[1]])
+    _, res, _ = orc.solve(prob, (0, 100), 0.1, 256)
+    nx = abs(res[1][N // 2, N // 2, -1]) ** 2
+    nc = abs(res[0][N // 2, N // 2, -1]) ** 2
+    r1 = abs(Wr - (dx + 1j * gx - g * nx) * (dc + 1j * gc) / Wr) ** 2 * nx / abs(A) ** 2
+    r2 = abs(dx + 1j * gx - g * nx) ** 2 * nx / Wr ** 2 / nc
+    assert abs(r1 - 1) < 3e-2 and abs(r2 - 1) < 3e-2
