@@ -60,14 +60,15 @@ def terms():
     return dict(dispersion=dispersion, nonlinearity=nonlinearity, pump=pump, param=polariton_param())
 
 
-def read_traffic(dtype):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    PMC summary (None when absent)."""
-    path = os.path.join(ROOT, "profiles", "c2_512sq_pmc_traffic.md")
+def read_traffic(dtype, kernel):
+    """HBM bytes per launch of `kernel` ("x_pass" / "y_pass") from the
+    committed rocprofv3 PMC summary (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "x_pass_y_pass_512sq_2comp_pmc.md")
     if not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch[{dtype},{kernel}]"
     for line in open(path):
-        if line.startswith(f"traffic_bytes_per_launch[{dtype}]"):
+        if line.strip().startswith(key):
             return float(line.split("=")[1].split()[0])
     return None
 
@@ -101,11 +102,72 @@ def gpu_run(g, torch, dtype, steps, warmup, dt=0.05):
     # per-kernel durations: HIP events around every launch of 200 more steps
     import ctypes
 
-    msx, msy = ctypes.c_double(), ctypes.c_double()
-    g._lib.check(g.load_library().gpe_profile_kernels(it.h, 200, ctypes.byref(msx), ctypes.byref(msy)))
+    msx, msy, mse = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    g._lib.check(g.load_library().gpe_profile_kernels(it.h, 200, ctypes.byref(msx), ctypes.byref(msy),
+                                                      ctypes.byref(mse)))
     finite = all(bool(torch.isfinite(r).all()) for r in res)
     it.close()
-    return dict(wall=wall, dev_ms=dev_ms, ms_x=msx.value, ms_y=msy.value, finite=finite)
+    # event-bracketed times minus the latency of an empty event pair
+    return dict(wall=wall, dev_ms=dev_ms, ms_x=msx.value - mse.value, ms_y=msy.value - mse.value,
+                ms_x_raw=msx.value, ms_y_raw=msy.value, ms_event=mse.value, finite=finite)
+
+
+def twa_run(g, torch, steps, batch=1024, n=256, dtype="complex64"):
+    """Batched truncated-Wigner ensemble: `batch` trajectories of the driven-
+    dissipative 1-component polariton on an n x n grid, L = 64 (dA = 1/16),
+    D = |k|^2/2 - delta - i gamma (delta = 0.5, gamma = 0.1), g = 1,
+    homogeneous pump F = 1.2, eta = sqrt(gamma / (2 dA)), Philox noise,
+    u0 = Wigner vacuum <|u0|^2> = 1/(2 dA), dt = 0.05, ensemble moments every
+    100 steps (no frames).  Returns timing and the last frame's mean density."""
+    L = 64.0
+    dA = (L / n) ** 2
+    gam, delta = 0.1, 0.5
+    shape = (n, n, batch)
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(0x5EED0003)
+    u = g.device_empty(shape, dtype, 0)
+    rdt = torch.float32 if dtype == "complex64" else torch.float64
+    re = torch.randn(tuple(reversed(shape)), generator=gen, device="cuda:0", dtype=rdt)
+    im = torch.randn(tuple(reversed(shape)), generator=gen, device="cuda:0", dtype=rdt)
+    u.copy_(torch.complex(re, im).permute(2, 1, 0) / math.sqrt(2) / math.sqrt(2 * dA))
+    del re, im
+    proto = (u,)
+    prob = g.GrossPitaevskiiProblem((u,), (L, L),
+                                    dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2 - delta - 1j * gam,
+                                    nonlinearity=lambda v, p: abs(v[0]) ** 2, pump=lambda r, p, t: 1.2 + 0 * r[0],
+                                    position_noise_func=lambda v, r, p: math.sqrt(gam / (2 * dA)),
+                                    noise_prototype=proto)
+    dt = 0.05
+    nsaves = max(1, steps // SAVE_EVERY)
+    it = g.init(prob, g.StrangSplitting(), (0.0, steps * dt), dt=dt, nsaves=nsaves, rng=g.PhiloxRNG(0x5EED0003))
+    it_w = g.init(prob, g.StrangSplitting(), (0.0, 20 * dt), dt=dt, nsaves=1, rng=g.PhiloxRNG(1))
+    it_w.solve(save_frames=False, ensemble_moments=True)
+    it_w.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ts, _, (n1, n2) = it.solve(save_frames=False, ensemble_moments=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    import ctypes
+
+    msx, msy, mse = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    g._lib.check(g.load_library().gpe_profile_kernels(it.h, 20, ctypes.byref(msx), ctypes.byref(msy),
+                                                      ctypes.byref(mse)))
+    it.close()
+    msx.value -= mse.value
+    msy.value -= mse.value
+    npts = n * n * batch
+    csz = 8 if dtype == "complex64" else 16
+    launch_bytes = 2 * csz * npts
+    dom = max(msx.value, msy.value)
+    return dict(workload=f"truncated-Wigner ensemble, {batch} x {n}x{n}, M=1, Philox noise, moments every 100 steps",
+                value=npts * steps / wall, unit="grid-point-steps/s", ms_per_step=wall * 1e3 / steps, steps=steps,
+                dtype="c64" if dtype == "complex64" else "c128",
+                hbm_equiv_frac=npts * steps / wall * 2 * 2 * csz / 1e9 / HBM_PEAK_GBS,
+                x_pass_us=msx.value * 1e3, y_pass_us=msy.value * 1e3,
+                dominant_kernel_GBs=launch_bytes / (dom * 1e-3) / 1e9,
+                dominant_kernel_frac=launch_bytes / (dom * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                mean_density_last_frame=float(n1[0][..., -1].mean()), vacuum_density=1 / (2 * dA))
 
 
 def cpu_baseline(seconds=12.0):
@@ -145,6 +207,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-twa", action="store_true", help="skip the batched truncated-Wigner line")
+    ap.add_argument("--twa-steps", type=int, default=2000)
     ap.add_argument("--dtype", default="complex64", choices=["complex64", "complex128"],
                     help="field precision of the headline run")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -207,11 +271,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": read_traffic(args.dtype),
+            "traffic": read_traffic(args.dtype, "y_pass" if dominant == "y" else "x_pass"),
             "avg_launch_us": dur_ms * 1e3,
             "algorithmic_bytes_per_launch": launch_bytes,
             "x_pass_us": head["ms_x"] * 1e3,
             "y_pass_us": head["ms_y"] * 1e3,
+            "timing": "HIP events around every launch on the library's stream, minus the mean latency of an "
+                      "empty event pair (%.2f us)" % (head["ms_event"] * 1e3),
         },
         "other_precision": {
             d: {"value": npts * args.steps / r["wall"], "ms_per_step": r["wall"] * 1e3 / args.steps,
@@ -222,6 +288,8 @@ def main():
         },
         "finite": all(r["finite"] for r in runs.values()),
     }
+    if not args.no_twa:
+        out["other_workloads"] = {"twa_1024x256sq": twa_run(g, torch, args.twa_steps)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -74,6 +74,7 @@ SIGNATURES = {
     "gpe_set_nonlinearity": (ctypes.c_int, [P, P, P]),
     "gpe_set_pump": (ctypes.c_int, [P, I32, P, P, I64]),
     "gpe_set_noise": (ctypes.c_int, [P, I32, P, ctypes.c_uint64, ctypes.c_uint64]),
+    "gpe_set_observables": (ctypes.c_int, [P, P, P]),
     "gpe_resolve_timestepping": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, D,
                                                  ctypes.POINTER(I64)]),
     "gpe_pump_times": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, P, I64]),
@@ -82,7 +83,7 @@ SIGNATURES = {
     "gpe_get_field": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "gpe_last_solve_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
     "gpe_advance": (ctypes.c_int, [P, I64, ctypes.POINTER(ctypes.c_float)]),
-    "gpe_profile_kernels": (ctypes.c_int, [P, I64, D, D]),
+    "gpe_profile_kernels": (ctypes.c_int, [P, I64, D, D, D]),
 }
 
 _lib = None

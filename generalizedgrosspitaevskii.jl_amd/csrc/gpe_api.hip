@@ -78,12 +78,18 @@ struct gpe_handle {
   int noise_kind = GPE_NOISE_NONE;
   double eta[kMaxM] = {0, 0};
   uint64_t seed = 0, call = 0;
-  // staging for frames that go to host memory
+  // staging for frames that go to host memory (or feed the observables only)
   void* stage[kMaxM] = {nullptr, nullptr};
+  // ensemble observables: caller's outputs and the device accumulation buffer
+  double* obs_n = nullptr;
+  double* obs_n2 = nullptr;
+  double* d_obs = nullptr;
+  size_t d_obs_cap = 0;
   // last solve configuration (for gpe_advance / timing)
   bool configured = false;
   double dt = 0;
   float last_ms = 0;
+  double event_pair_ms = 0;  // latency of an empty HIP event pair (gpe_profile_kernels)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -231,6 +237,8 @@ extern "C" int gpe_destroy(gpe_handle* h) {
     dfree(h->stage[c]);
   }
   dfree(h->tw);
+  void* po = h->d_obs;
+  dfree(po);
   void* p = h->dD;
   dfree(p);
   p = h->dV;
@@ -612,11 +620,21 @@ struct FrameSink {
   char* host_dst[kMaxM] = {nullptr, nullptr};
 };
 
+extern "C" int gpe_set_observables(gpe_handle* h, double* n, double* n2) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_observables: null handle");
+  if ((n == nullptr) != (n2 == nullptr)) return fail(GPE_ERR_INVALID, "gpe_set_observables: give both n and n2, or neither");
+  h->obs_n = n;
+  h->obs_n2 = n2;
+  return GPE_OK;
+}
+
 extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double dt, int64_t nsaves,
                          int32_t save_start, void* const* result, double* ts) {
-  if (!h || !u0 || !result) return fail(GPE_ERR_INVALID, "gpe_solve: null argument");
-  for (int c = 0; c < h->M; ++c)
-    if (!u0[c] || !result[c]) return fail(GPE_ERR_INVALID, "gpe_solve: null field pointer for component %d", c);
+  if (!h || !u0) return fail(GPE_ERR_INVALID, "gpe_solve: null argument");
+  for (int c = 0; c < h->M; ++c) {
+    if (!u0[c]) return fail(GPE_ERR_INVALID, "gpe_solve: null field pointer for component %d", c);
+    if (result && !result[c]) return fail(GPE_ERR_INVALID, "gpe_solve: null result pointer for component %d", c);
+  }
   HIPCHK(hipSetDevice(h->d.device));
   double rdt;
   int64_t sps;
@@ -643,19 +661,63 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
   const size_t fb = field_bytes(h);
   for (int c = 0; c < h->M; ++c) HIPCHK(hipMemcpyAsync(h->u[c], u0[c], fb, hipMemcpyDefault, h->stream));
   const int64_t nfr = nsaves + (save_start ? 1 : 0);
-  bool res_dev = true;
-  for (int c = 0; c < h->M; ++c) res_dev = res_dev && is_device_ptr(result[c]);
+  const bool obs = h->obs_n != nullptr;
+  bool res_dev = result != nullptr;
+  for (int c = 0; c < h->M && res_dev; ++c) res_dev = is_device_ptr(result[c]);
   if (!res_dev) {
     rc = ensure_stage(h);
     if (rc) return rc;
   }
-  if (save_start)
+  const size_t obs_per_frame = (size_t)h->M * h->nsp;
+  if (obs) {
+    const size_t need = 2 * obs_per_frame * nfr;
+    if (need > h->d_obs_cap) {
+      void* p = h->d_obs;
+      dfree(p);
+      h->d_obs = nullptr;
+      HIPCHK(hipMalloc(&p, need * sizeof(double)));
+      h->d_obs = (double*)p;
+      h->d_obs_cap = need;
+    }
+  }
+  // observables of frame f from the real-space field at `src`
+  auto observe = [&](void* const* src, int64_t f) -> int {
+    if (!obs) return GPE_OK;
+    ObsArgs oa;
+    memset(&oa, 0, sizeof oa);
+    for (int c = 0; c < h->M; ++c) {
+      oa.u[c] = src[c];
+      oa.n1[c] = h->d_obs + ((size_t)c * nfr + f) * h->nsp;
+      oa.n2[c] = h->d_obs + obs_per_frame * nfr + ((size_t)c * nfr + f) * h->nsp;
+    }
+    oa.nsp = h->nsp;
+    oa.batch = h->batch;
+    oa.M = h->M;
+    const unsigned blocks = (unsigned)((h->nsp + 255) / 256);
+    if (h->f64)
+      hipLaunchKernelGGL(observables_kernel<double>, dim3(blocks), dim3(256), 0, h->stream, oa);
+    else
+      hipLaunchKernelGGL(observables_kernel<float>, dim3(blocks), dim3(256), 0, h->stream, oa);
+    HIPCHK(hipGetLastError());
+    return GPE_OK;
+  };
+  if (save_start && result)
     for (int c = 0; c < h->M; ++c) HIPCHK(hipMemcpyAsync(result[c], u0[c], fb, hipMemcpyDefault, h->stream));
+  if (save_start) {
+    rc = observe(h->u, 0);
+    if (rc) return rc;
+  }
+  const bool want_frames = result != nullptr || obs;
   auto frame_ptr = [&](int c, int64_t f) -> void* {
+    if (!want_frames) return nullptr;
     return res_dev ? (void*)((char*)result[c] + (size_t)f * fb) : h->stage[c];
   };
   auto flush_frame = [&](int64_t f) -> int {
-    if (res_dev) return GPE_OK;
+    void* src[kMaxM];
+    for (int c = 0; c < h->M; ++c) src[c] = frame_ptr(c, f);
+    int r = observe(src, f);
+    if (r) return r;
+    if (res_dev || !result) return GPE_OK;
     for (int c = 0; c < h->M; ++c)
       HIPCHK(hipMemcpyAsync((char*)result[c] + (size_t)f * fb, h->stage[c], fb, hipMemcpyDeviceToHost, h->stream));
     return GPE_OK;
@@ -707,11 +769,15 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
   }
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   h->call = call0 + 2 * (uint64_t)S;
+  if (obs) {
+    HIPCHK(hipMemcpyAsync(h->obs_n, h->d_obs, obs_per_frame * nfr * sizeof(double), hipMemcpyDefault, h->stream));
+    HIPCHK(hipMemcpyAsync(h->obs_n2, h->d_obs + obs_per_frame * nfr, obs_per_frame * nfr * sizeof(double),
+                          hipMemcpyDefault, h->stream));
+  }
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
   h->configured = true;
   h->dt = rdt;
-  (void)nfr;
   return GPE_OK;
 }
 
@@ -819,6 +885,24 @@ static int advance_impl(gpe_handle* h, int64_t nsteps, bool per_kernel, double* 
       cnt[kinds[i]]++;
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    // latency of an event pair with nothing between it, on the same stream
+    double empty = 0;
+    const int npair = 64;
+    std::vector<hipEvent_t> pe(2 * npair);
+    for (auto& e : pe) HIPCHK(hipEventCreate(&e));
+    for (int i = 0; i < npair; ++i) {
+      HIPCHK(hipEventRecord(pe[2 * i], h->stream));
+      HIPCHK(hipEventRecord(pe[2 * i + 1], h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < npair; ++i) {
+      float ms;
+      HIPCHK(hipEventElapsedTime(&ms, pe[2 * i], pe[2 * i + 1]));
+      empty += ms;
+    }
+    for (auto& e : pe) (void)hipEventDestroy(e);
+    empty /= npair;
+    h->event_pair_ms = empty;
     if (ms_x) *ms_x = h->ndim == 2 ? (cnt[0] ? acc[0] / cnt[0] : 0) : (cnt[2] ? acc[2] / cnt[2] : 0);
     if (ms_y) *ms_y = cnt[1] ? acc[1] / cnt[1] : 0;
   }
@@ -833,7 +917,10 @@ extern "C" int gpe_advance(gpe_handle* h, int64_t nsteps, float* ms) {
   return GPE_OK;
 }
 
-extern "C" int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y) {
+extern "C" int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y, double* ms_event) {
   if (!h) return fail(GPE_ERR_INVALID, "gpe_profile_kernels: null handle");
-  return advance_impl(h, nsteps, true, ms_x, ms_y);
+  int rc = advance_impl(h, nsteps, true, ms_x, ms_y);
+  if (rc) return rc;
+  if (ms_event) *ms_event = h->event_pair_ms;
+  return GPE_OK;
 }

@@ -628,4 +628,55 @@ __global__ void ev_table_kernel(EvArgs a) {
   }
 }
 
+// ------------------------------------------------- ensemble observables
+// For every spatial point r and component c: the batch means of |u|^2 and
+// |u|^4 of one saved frame, accumulated in FP64 with Neumaier compensation
+// (examples/truncated_wigner.jl:110-165 computes these moments after the
+// solve in user code).  One thread per point; the batch loop reads
+// consecutive points at fixed trajectory, so loads are coalesced.
+struct ObsArgs {
+  const void* u[kMaxM];  // real-space frame, nsp * batch values per component
+  double* n1[kMaxM];     // nsp outputs per component (this frame)
+  double* n2[kMaxM];
+  int64_t nsp, batch;
+  int M;
+};
+
+__device__ __forceinline__ void neumaier_add(double& s, double& comp, double x) {
+  const double t = s + x;
+  comp += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+
+template <typename R>
+__global__ void observables_kernel(ObsArgs a) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.nsp) return;
+  const double inv = 1.0 / (double)a.batch;
+  for (int c = 0; c < a.M; ++c) {
+    const Cx<R>* u = reinterpret_cast<const Cx<R>*>(a.u[c]) + r;
+    double s1 = 0, c1 = 0, s2 = 0, c2 = 0;
+    int64_t b = 0;
+    for (; b + 4 <= a.batch; b += 4) {
+      Cx<R> v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = u[(b + i) * a.nsp];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double x = (double)v[i].re * (double)v[i].re + (double)v[i].im * (double)v[i].im;
+        neumaier_add(s1, c1, x);
+        neumaier_add(s2, c2, x * x);
+      }
+    }
+    for (; b < a.batch; ++b) {
+      const Cx<R> v = u[b * a.nsp];
+      const double x = (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+      neumaier_add(s1, c1, x);
+      neumaier_add(s2, c2, x * x);
+    }
+    a.n1[c][r] = (s1 + c1) * inv;
+    a.n2[c][r] = (s2 + c2) * inv;
+  }
+}
+
 }  // namespace gpe

@@ -520,30 +520,54 @@ class StrangSplittingIterator:
             pass
 
     # ------------------------------------------------------------ solve!
-    def solve(self, result=None):
-        """solve!(iter) (src/fixed_time_stepping.jl:26-54): returns (ts, result)."""
+    def solve(self, result=None, save_frames=True, ensemble_moments=False):
+        """solve!(iter) (src/fixed_time_stepping.jl:26-54): returns (ts, result).
+
+        Extensions: ``save_frames=False`` returns result = None and writes no
+        frames; ``ensemble_moments=True`` also returns (n, n2), the batch means
+        of |u_c|^2 and |u_c|^4 of every saved frame (FP64, reduced on the
+        device), one array of shape spatial + (nframes,) per component."""
         prob, lib = self.prob, self._lib
         nfr = self.nsaves + int(self.save_start)
         rshape = self.shape + (nfr,)
         if self.on_device:
             u0 = [x if _contig_colmajor(x) else _to_colmajor(x) for x in prob.u0]
-            if result is None:
+            if result is None and save_frames:
                 result = tuple(device_empty(rshape, str(x.dtype).replace("torch.", ""), x.device.index) for x in u0)
             uptr = [x.data_ptr() for x in u0]
-            rptr = [x.data_ptr() for x in result]
+            rptr = None if result is None else [x.data_ptr() for x in result]
         else:
             u0 = [np.asfortranarray(x) for x in prob.u0]
-            if result is None:
+            if result is None and save_frames:
                 result = tuple(np.empty(rshape, dtype=x.dtype, order="F") for x in u0)
             uptr = [x.ctypes.data for x in u0]
-            rptr = [x.ctypes.data for x in result]
+            rptr = None if result is None else [x.ctypes.data for x in result]
+        moments = None
+        if ensemble_moments:
+            mshape = (prob.M,) + tuple(self.spatial) + (nfr,)
+            n1 = np.empty((prob.M, int(np.prod(self.spatial)) * nfr), dtype=np.float64)
+            n2 = np.empty_like(n1)
+            L.check(lib.gpe_set_observables(self.h, n1.ctypes.data, n2.ctypes.data))
+        else:
+            L.check(lib.gpe_set_observables(self.h, None, None))
         ts = np.empty(nfr, dtype=np.float64)
-        L.check(lib.gpe_solve(self.h, L.ptr_array(uptr), self.tspan[0], self.tspan[1], self.dt_in, self.nsaves,
-                              int(self.save_start), L.ptr_array(rptr), ts.ctypes.data))
+        try:
+            L.check(lib.gpe_solve(self.h, L.ptr_array(uptr), self.tspan[0], self.tspan[1], self.dt_in, self.nsaves,
+                                  int(self.save_start), None if rptr is None else L.ptr_array(rptr),
+                                  ts.ctypes.data))
+        finally:
+            if ensemble_moments:
+                lib.gpe_set_observables(self.h, None, None)
         if prob.noise_prototype is not None:
             self.rng.counter += 2 * self.steps_per_save * self.nsaves
         self._u0_keep = u0
-        return ts, tuple(result)
+        out_res = None if result is None else tuple(result)
+        if ensemble_moments:
+            shape_f = tuple(self.spatial) + (nfr,)
+            moments = (tuple(n1[c].reshape(shape_f, order="F") for c in range(prob.M)),
+                       tuple(n2[c].reshape(shape_f, order="F") for c in range(prob.M)))
+            return ts, out_res, moments
+        return ts, out_res
 
     def last_ms(self):
         ms = ctypes.c_float()
@@ -571,10 +595,11 @@ def init(prob, alg, tspan, *, dt, nsaves, show_progress=False, progress=None, sa
 
 
 def solve(prob, alg, tspan, *, dt, nsaves, show_progress=False, progress=None, save_start=True,
-          workgroup_size=(), rng=None):
-    """src/fixed_time_stepping.jl:79-81: solve!(init(prob, alg, tspan; kwargs...))."""
+          workgroup_size=(), rng=None, save_frames=True, ensemble_moments=False):
+    """src/fixed_time_stepping.jl:79-81: solve!(init(prob, alg, tspan; kwargs...)).
+    save_frames / ensemble_moments: see StrangSplittingIterator.solve."""
     it = init(prob, alg, tspan, dt=dt, nsaves=nsaves, save_start=save_start, rng=rng)
     try:
-        return it.solve()
+        return it.solve(save_frames=save_frames, ensemble_moments=ensemble_moments)
     finally:
         it.close()

@@ -110,6 +110,17 @@ int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, const doubl
  * draws per step).  kind GPE_NOISE_NONE disables noise. */
 int gpe_set_noise(gpe_handle* h, int32_t kind, const double* eta, uint64_t seed, uint64_t call0);
 
+/* Ensemble observables of every saved frame of the next gpe_solve calls,
+ * reduced on the device over the batch (trajectory) dimension in FP64 with
+ * compensated summation: for component c, frame f and spatial point r
+ *     n [(c*nframes + f)*prod(n) + r] = mean_b |u_c(r, b)|^2
+ *     n2[(c*nframes + f)*prod(n) + r] = mean_b |u_c(r, b)|^4
+ * (host or device memory, nframes = nsaves + save_start).  The reference has
+ * no such entry point: its users compute these moments from the returned
+ * frames (examples/truncated_wigner.jl:110-165, test/windowed_ft.jl:31-49).
+ * n == NULL disables. */
+int gpe_set_observables(gpe_handle* h, double* n, double* n2);
+
 /* resolve_fixed_timestepping (src/fixed_time_stepping.jl:14-24). */
 int gpe_resolve_timestepping(double dt, double t0, double t1, int64_t nsaves, double* dt_out,
                              int64_t* steps_per_save);
@@ -123,7 +134,8 @@ int gpe_pump_times(double t0, double t1, double dt, int64_t nsaves, double* time
 /* The whole fixed-step solve: init + solve! (src/strang_splitting.jl:32-67,
  * src/fixed_time_stepping.jl:26-54, src/strang_splitting.jl:86-90).
  *   u0[c]     : M pointers, field arrays (prod(n) * batch complex values)
- *   result[c] : M pointers, arrays of shape (n..., batch, nsaves + save_start)
+ *   result[c] : M pointers, arrays of shape (n..., batch, nsaves + save_start);
+ *               result == NULL skips the frames (observables-only runs)
  *   ts        : nsaves + save_start doubles (may be NULL)
  * Returns after result and ts are written. */
 int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double dt,
@@ -142,9 +154,11 @@ int gpe_last_solve_ms(gpe_handle* h, float* ms);
 int gpe_advance(gpe_handle* h, int64_t nsteps, float* ms);
 
 /* As gpe_advance, with HIP events around every kernel launch: *ms_x is the
- * mean duration in ms of the fused real-space kernel (2D x-pass, or the 1D
- * whole-step kernel), *ms_y that of the 2D k-space y-pass (0 in 1D). */
-int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y);
+ * mean event-bracketed time in ms of the fused real-space kernel (2D x-pass,
+ * or the 1D whole-step kernel), *ms_y that of the 2D k-space y-pass (0 in
+ * 1D), and *ms_event (may be NULL) the mean time of an empty event pair on
+ * the same stream, the bracketing's own latency. */
+int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y, double* ms_event);
 
 #ifdef __cplusplus
 }

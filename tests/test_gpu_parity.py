@@ -215,3 +215,78 @@ def test_save_start_false_and_frames(ggp, orc):
     u0, lengths = c1_inputs(64)
     out = run_both(ggp, orc, u0, lengths, (0, 0.2), 0.01, 5, save_start=False, dispersion=disp_kin)
     check(*out, FP64_TOL)
+
+
+# ------------------------------------------ general two-component forms
+def test_two_component_nonsymmetric_coupled(ggp, orc):
+    """Full non-symmetric 2x2 D(k) with a k-linear (odd) coupling, so neither
+    the symmetric nor the folded table applies; cross-phase nonlinearity
+    couples |u_1|^2 and |u_2|^2; noise on both components."""
+    N, B, L = 32, 2, 8.0
+    rng = np.random.default_rng(5)
+    u0 = tuple(rng.standard_normal((N, N, B)) * 0.3 + 1j * rng.standard_normal((N, N, B)) * 0.3 for _ in range(2))
+
+    def disp(k, p):
+        kk = k[0] ** 2 + k[1] ** 2
+        return [[kk / 2 - 0.1j, 0.4 * k[0] + 0.2], [0.4 * k[0] - 0.3j, kk / 3 - 0.05j]]
+
+    out = run_both(ggp, orc, u0, (L, L), (0, 0.6), 0.05, 3, seed=77, dispersion=disp,
+                   nonlinearity=lambda u, p: [abs(u[0]) ** 2 + 0.5 * abs(u[1]) ** 2,
+                                              0.3 * abs(u[0]) ** 2 + abs(u[1]) ** 2 - 0.02j],
+                   position_noise_func=lambda u, r, p: [0.1, 0.2], noise_prototype=True)
+    check(*out, FP64_TOL)
+
+
+def test_two_component_1d_full_dispersion(ggp, orc):
+    N, B, L = 128, 4, 30.0
+    rng = np.random.default_rng(6)
+    u0 = tuple(rng.standard_normal((N, B)) + 1j * rng.standard_normal((N, B)) for _ in range(2))
+    out = run_both(ggp, orc, u0, (L,), (0, 1.0), 0.02, 4, **polariton_terms(L))
+    check(*out, FP64_TOL)
+
+
+# ------------------------------------------------ ensemble observables
+def test_ensemble_moments_match_oracle(ggp, orc):
+    N, B, L = 32, 16, 8.0
+    u0 = (np.full((N, N, B), 0.3 + 0.1j),)
+    terms = dict(dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2 - 0.5 - 0.05j,
+                 nonlinearity=lambda u, p: abs(u[0]) ** 2, pump=lambda r, p, t: 1.2 + 0 * r[0],
+                 position_noise_func=lambda u, r, p: 0.3)
+    ts_o, res_o, _ = orc.solve(orc.Problem(u0, (L, L), noise_prototype="complex", noise_seed=5, **terms),
+                               (0, 2.0), 0.05, 4)
+    prob = ggp.GrossPitaevskiiProblem(u0, (L, L), noise_prototype=(np.empty((N, N, B), complex),), **terms)
+    for frames in (True, False):
+        ts, res, (n1, n2) = ggp.solve(prob, ggp.StrangSplitting(), (0, 2.0), dt=0.05, nsaves=4,
+                                      rng=ggp.PhiloxRNG(5), save_frames=frames, ensemble_moments=True)
+        assert (res is None) != frames
+        a2 = abs(res_o[0]) ** 2
+        want1 = a2.mean(axis=2)
+        want2 = (a2 ** 2).mean(axis=2)
+        assert n1[0].shape == want1.shape
+        assert np.max(abs(n1[0] - want1)) <= 1e-12 * np.max(want1)
+        assert np.max(abs(n2[0] - want2)) <= 1e-12 * np.max(want2)
+        if frames:
+            assert rel_l2(res[0], res_o[0]) <= FP64_TOL
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_twa_stationary_vacuum_occupation(ggp, dtype):
+    """Truncated Wigner vacuum: D = |k|^2/2 - delta - i gamma/2 with
+    eta = sqrt(gamma / (2 dA)) keeps <|psi|^2> = 1/(2 dA) (the pairing of
+    test/windowed_ft.jl:24,28), starting from the Wigner vacuum itself."""
+    N, B, L = 64, 256, 16.0
+    dA = (L / N) ** 2
+    gam, delta = 0.1, 0.5
+    rng = np.random.default_rng(9)
+    u0 = ((rng.standard_normal((N, N, B)) + 1j * rng.standard_normal((N, N, B))) / math.sqrt(2) / math.sqrt(2 * dA),)
+    prob = ggp.GrossPitaevskiiProblem(tuple(x.astype(dtype) for x in u0), (L, L),
+                                      dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2 - delta - 0.5j * gam,
+                                      position_noise_func=lambda u, r, p: math.sqrt(gam / (2 * dA)),
+                                      noise_prototype=(np.empty((N, N, B), dtype),))
+    ts, res, (n1, n2) = ggp.solve(prob, ggp.StrangSplitting(), (0, 40.0), dt=0.05, nsaves=4,
+                                  rng=ggp.PhiloxRNG(2024), save_frames=False, ensemble_moments=True)
+    target = 1 / (2 * dA)
+    for f in range(n1[0].shape[-1]):
+        assert abs(n1[0][..., f].mean() / target - 1) < 0.02
+        # complex Gaussian field: <|psi|^4> = 2 <|psi|^2>^2
+        assert abs(n2[0][..., f].mean() / (2 * n1[0][..., f].mean() ** 2) - 1) < 0.03
