@@ -84,6 +84,14 @@ SIGNATURES = {
     "gpe_last_solve_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
     "gpe_advance": (ctypes.c_int, [P, I64, ctypes.POINTER(ctypes.c_float)]),
     "gpe_profile_kernels": (ctypes.c_int, [P, I64, D, D, D]),
+    "gpe_sweep_create": (ctypes.c_int, [I64, P, P, I32, I32, ctypes.POINTER(P)]),
+    "gpe_sweep_destroy": (ctypes.c_int, [P]),
+    "gpe_sweep_set_dispersion": (ctypes.c_int, [P, P]),
+    "gpe_sweep_set_nonlinearity": (ctypes.c_int, [P, P, P]),
+    "gpe_sweep_set_pump": (ctypes.c_int, [P, P, P, I64]),
+    "gpe_sweep_set_noise": (ctypes.c_int, [P, I32, P, ctypes.c_uint64, ctypes.c_uint64]),
+    "gpe_sweep_solve": (ctypes.c_int, [P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, I32, P, P]),
+    "gpe_sweep_last_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None

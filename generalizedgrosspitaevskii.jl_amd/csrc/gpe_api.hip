@@ -7,7 +7,9 @@
 //   step! ordering               src/strang_splitting.jl:86-90
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -922,5 +924,327 @@ extern "C" int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, 
   int rc = advance_impl(h, nsteps, true, ms_x, ms_y);
   if (rc) return rc;
   if (ms_event) *ms_event = h->event_pair_ms;
+  return GPE_OK;
+}
+
+// ------------------------------------------------------------------ sweeps
+// Independent 1D single-component instances of uneven length, advanced
+// together (one launch per distinct length and save block).
+struct gpe_sweep {
+  int32_t device = 0;
+  bool f64 = false;
+  size_t csz = 8;
+  int64_t ninst = 0, total = 0, nmax = 0;
+  std::vector<int64_t> n, off;
+  std::vector<double> L;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0;
+  void* u = nullptr;          // concatenated field
+  void* tw = nullptr;         // twiddles of length nmax
+  double* dD = nullptr;       // FP64 D tables
+  void* ed = nullptr;         // exp(-i dt D) (field precision)
+  double ed_dt = NAN;
+  void* pump = nullptr;       // concatenated profiles (field precision) or null
+  std::vector<int64_t> pump_off;
+  std::vector<double> env;
+  double* d_env = nullptr;
+  int64_t d_env_cap = 0;
+  std::vector<double> g, c;   // complex per instance
+  int noise_kind = GPE_NOISE_NONE;
+  std::vector<double> eta;
+  uint64_t seed = 0, call = 0;
+  SweepInst* d_inst = nullptr;
+  std::vector<int64_t> sizes;                 // distinct lengths
+  std::vector<int32_t*> d_ids;                // instance ids per length
+  std::vector<int64_t> nids;
+  void* d_res = nullptr;      // device result staging when the caller's result is on the host
+  size_t d_res_cap = 0;
+};
+
+extern "C" int gpe_sweep_destroy(gpe_sweep* s) {
+  if (!s) return GPE_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  dfree(s->u);
+  dfree(s->tw);
+  void* p = s->dD;
+  dfree(p);
+  dfree(s->ed);
+  dfree(s->pump);
+  p = s->d_env;
+  dfree(p);
+  p = s->d_inst;
+  dfree(p);
+  for (auto& q : s->d_ids) {
+    void* v = q;
+    dfree(v);
+  }
+  dfree(s->d_res);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return GPE_OK;
+}
+
+extern "C" int gpe_sweep_create(int64_t ninst, const int64_t* n, const double* L, int32_t dtype, int32_t device,
+                                gpe_sweep** out) {
+  if (!out || !n || !L || ninst < 1) return fail(GPE_ERR_INVALID, "gpe_sweep_create: bad argument");
+  *out = nullptr;
+  if (dtype != GPE_C64 && dtype != GPE_C128) return fail(GPE_ERR_INVALID, "gpe_sweep_create: dtype=%d", dtype);
+  for (int64_t i = 0; i < ninst; ++i) {
+    if (!is_pow2(n[i]) || n[i] < 16 || n[i] > 4096)
+      return fail(GPE_ERR_UNSUPPORTED, "gpe_sweep_create: n[%lld]=%lld (supported: powers of two 16..4096)",
+                  (long long)i, (long long)n[i]);
+    if (!(L[i] > 0) || !std::isfinite(L[i])) return fail(GPE_ERR_INVALID, "gpe_sweep_create: L[%lld]", (long long)i);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GPE_ERR_DEVICE, "gpe_sweep_create: no HIP device");
+  if (device < 0 || device >= ndev) return fail(GPE_ERR_INVALID, "gpe_sweep_create: device=%d", device);
+  HIPCHK(hipSetDevice(device));
+  gpe_sweep* s = new gpe_sweep();
+  auto bail = [&](int code) {
+    gpe_sweep_destroy(s);
+    return code;
+  };
+  s->device = device;
+  s->f64 = dtype == GPE_C128;
+  s->csz = s->f64 ? 16 : 8;
+  s->ninst = ninst;
+  s->n.assign(n, n + ninst);
+  s->L.assign(L, L + ninst);
+  s->off.resize(ninst);
+  for (int64_t i = 0; i < ninst; ++i) {
+    s->off[i] = s->total;
+    s->total += n[i];
+    s->nmax = std::max(s->nmax, n[i]);
+  }
+  s->g.assign(2 * ninst, 0.0);
+  s->c.assign(2 * ninst, 0.0);
+  s->eta.assign(ninst, 0.0);
+  s->pump_off.assign(ninst, -1);
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return bail(fail(GPE_ERR_DEVICE, "gpe_sweep_create: stream/events"));
+  if (hipMalloc(&s->u, (size_t)s->total * s->csz) != hipSuccess)
+    return bail(fail(GPE_ERR_NOMEM, "gpe_sweep_create: field of %lld points", (long long)s->total));
+  std::vector<double> w;
+  twiddles_fp64(s->nmax, w);
+  int rc;
+  if (s->f64) {
+    rc = upload(&s->tw, w.data(), w.size() * sizeof(double));
+  } else {
+    std::vector<float> wf(w.begin(), w.end());
+    rc = upload(&s->tw, wf.data(), wf.size() * sizeof(float));
+  }
+  if (rc) return bail(rc);
+  // instance ids grouped by length
+  for (int64_t i = 0; i < ninst; ++i)
+    if (std::find(s->sizes.begin(), s->sizes.end(), n[i]) == s->sizes.end()) s->sizes.push_back(n[i]);
+  std::sort(s->sizes.begin(), s->sizes.end(), std::greater<int64_t>());
+  for (int64_t N : s->sizes) {
+    std::vector<int32_t> ids;
+    for (int64_t i = 0; i < ninst; ++i)
+      if (n[i] == N) ids.push_back((int32_t)i);
+    void* p = nullptr;
+    rc = upload(&p, ids.data(), ids.size() * sizeof(int32_t));
+    if (rc) return bail(rc);
+    s->d_ids.push_back((int32_t*)p);
+    s->nids.push_back((int64_t)ids.size());
+  }
+  *out = s;
+  return GPE_OK;
+}
+
+extern "C" int gpe_sweep_set_dispersion(gpe_sweep* s, const double* D) {
+  if (!s || !D) return fail(GPE_ERR_INVALID, "gpe_sweep_set_dispersion: null argument");
+  HIPCHK(hipSetDevice(s->device));
+  void* p = s->dD;
+  int rc = upload(&p, D, (size_t)s->total * 2 * sizeof(double));
+  s->dD = (double*)p;
+  if (rc) return rc;
+  if (!s->ed) HIPCHK(hipMalloc(&s->ed, (size_t)s->total * s->csz));
+  s->ed_dt = NAN;
+  return GPE_OK;
+}
+
+extern "C" int gpe_sweep_set_nonlinearity(gpe_sweep* s, const double* g, const double* c) {
+  if (!s) return fail(GPE_ERR_INVALID, "gpe_sweep_set_nonlinearity: null handle");
+  for (int64_t i = 0; i < 2 * s->ninst; ++i) {
+    s->g[i] = g ? g[i] : 0.0;
+    s->c[i] = c ? c[i] : 0.0;
+  }
+  return GPE_OK;
+}
+
+extern "C" int gpe_sweep_set_pump(gpe_sweep* s, const double* profile, const double* envelope, int64_t n_env) {
+  if (!s) return fail(GPE_ERR_INVALID, "gpe_sweep_set_pump: null handle");
+  if (n_env < 0 || (n_env > 0 && !envelope)) return fail(GPE_ERR_INVALID, "gpe_sweep_set_pump: envelope");
+  HIPCHK(hipSetDevice(s->device));
+  dfree(s->pump);
+  s->pump_off.assign(s->ninst, -1);
+  s->env.clear();
+  if (!profile) return GPE_OK;
+  for (int64_t i = 0; i < s->ninst; ++i) {
+    bool nz = false;
+    for (int64_t k = 0; k < 2 * s->n[i] && !nz; ++k) nz = profile[2 * s->off[i] + k] != 0.0;
+    if (nz) s->pump_off[i] = s->off[i];
+  }
+  int rc;
+  if (s->f64) {
+    rc = upload(&s->pump, profile, (size_t)s->total * 16);
+  } else {
+    std::vector<float> f(2 * s->total);
+    for (int64_t k = 0; k < 2 * s->total; ++k) f[k] = (float)profile[k];
+    rc = upload(&s->pump, f.data(), (size_t)s->total * 8);
+  }
+  if (rc) return rc;
+  s->env.assign(envelope, envelope + 2 * n_env);
+  return GPE_OK;
+}
+
+extern "C" int gpe_sweep_set_noise(gpe_sweep* s, int32_t kind, const double* eta, uint64_t seed, uint64_t call0) {
+  if (!s) return fail(GPE_ERR_INVALID, "gpe_sweep_set_noise: null handle");
+  if (kind != GPE_NOISE_NONE && kind != GPE_NOISE_COMPLEX && kind != GPE_NOISE_REAL)
+    return fail(GPE_ERR_INVALID, "gpe_sweep_set_noise: kind=%d", kind);
+  if (kind != GPE_NOISE_NONE && !eta) return fail(GPE_ERR_INVALID, "gpe_sweep_set_noise: null eta");
+  s->noise_kind = kind;
+  for (int64_t i = 0; i < s->ninst; ++i) s->eta[i] = kind != GPE_NOISE_NONE ? eta[i] : 0.0;
+  s->seed = seed;
+  s->call = call0;
+  return GPE_OK;
+}
+
+extern "C" int gpe_sweep_solve(gpe_sweep* s, const void* u0, double t0, double t1, double dt, int64_t nsaves,
+                               int32_t save_start, void* result, double* ts) {
+  if (!s || !u0) return fail(GPE_ERR_INVALID, "gpe_sweep_solve: null argument");
+  if (!s->dD) return fail(GPE_ERR_STATE, "gpe_sweep_solve: no dispersion set");
+  HIPCHK(hipSetDevice(s->device));
+  double rdt;
+  int64_t sps;
+  int rc = gpe_resolve_timestepping(dt, t0, t1, nsaves, &rdt, &sps);
+  if (rc) return rc;
+  const int64_t S = sps * nsaves;
+  if (!s->env.empty() && (int64_t)s->env.size() != 2 * (1 + 2 * S))
+    return fail(GPE_ERR_STATE, "gpe_sweep_solve: envelope has %lld samples, this solve needs %lld",
+                (long long)(s->env.size() / 2), (long long)(1 + 2 * S));
+  if (!s->env.empty()) {
+    const int64_t need = (int64_t)s->env.size();
+    if (need > s->d_env_cap) {
+      void* p = s->d_env;
+      dfree(p);
+      s->d_env = nullptr;
+      HIPCHK(hipMalloc(&p, need * sizeof(double)));
+      s->d_env = (double*)p;
+      s->d_env_cap = need;
+    }
+    HIPCHK(hipMemcpyAsync(s->d_env, s->env.data(), need * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  }
+  if (!(s->ed_dt == rdt)) {
+    EdArgs a;
+    a.D = s->dD;
+    a.out = s->ed;
+    a.npts = s->total;
+    a.form = 1;
+    a.M = 1;
+    a.dt = rdt;
+    a.scale = 1.0;
+    const unsigned blocks = (unsigned)((s->total + 255) / 256);
+    if (s->f64)
+      hipLaunchKernelGGL(ed_table_kernel<double>, dim3(blocks), dim3(256), 0, s->stream, a);
+    else
+      hipLaunchKernelGGL(ed_table_kernel<float>, dim3(blocks), dim3(256), 0, s->stream, a);
+    HIPCHK(hipGetLastError());
+    s->ed_dt = rdt;
+  }
+  const int64_t nfr = nsaves + (save_start ? 1 : 0);
+  // instance records
+  std::vector<SweepInst> inst(s->ninst);
+  bool g_cplx = false;
+  for (int64_t i = 0; i < s->ninst; ++i) {
+    inst[i].off = s->off[i];
+    inst[i].pump_off = s->pump_off[i];
+    inst[i].res_off = s->off[i] * nfr;
+    inst[i].g[0] = s->g[2 * i];
+    inst[i].g[1] = s->g[2 * i + 1];
+    inst[i].c[0] = s->c[2 * i];
+    inst[i].c[1] = s->c[2 * i + 1];
+    inst[i].eta = s->eta[i];
+    g_cplx = g_cplx || inst[i].g[1] != 0.0 || inst[i].c[1] != 0.0;
+  }
+  if (!s->d_inst) HIPCHK(hipMalloc((void**)&s->d_inst, s->ninst * sizeof(SweepInst)));
+  HIPCHK(hipMemcpyAsync(s->d_inst, inst.data(), s->ninst * sizeof(SweepInst), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->u, u0, (size_t)s->total * s->csz, hipMemcpyDefault, s->stream));
+  const size_t res_bytes = (size_t)s->total * nfr * s->csz;
+  void* res = nullptr;
+  bool res_host = false;
+  if (result) {
+    if (is_device_ptr(result)) {
+      res = result;
+    } else {
+      res_host = true;
+      if (res_bytes > s->d_res_cap) {
+        dfree(s->d_res);
+        HIPCHK(hipMalloc(&s->d_res, res_bytes));
+        s->d_res_cap = res_bytes;
+      }
+      res = s->d_res;
+    }
+  }
+  SweepArgs a;
+  memset(&a, 0, sizeof a);
+  a.u = s->u;
+  a.res = res;
+  a.inst = s->d_inst;
+  a.env = s->env.empty() ? nullptr : s->d_env;
+  a.delta = rdt / 2;
+  a.ed = s->ed;
+  a.pump = s->pump;
+  a.noise_kind = s->noise_kind;
+  a.seed_lo = (uint32_t)s->seed;
+  a.seed_hi = (uint32_t)(s->seed >> 32);
+  a.g_cplx = g_cplx ? 1 : 0;
+  auto launch_all = [&](int64_t nsteps, int64_t frame, int64_t env_base, uint64_t call0) -> int {
+    for (size_t k = 0; k < s->sizes.size(); ++k) {
+      SweepArgs b = a;
+      b.ids = s->d_ids[k];
+      b.ninst = s->nids[k];
+      b.nsteps = nsteps;
+      b.frame = frame;
+      b.env_base = env_base;
+      b.call0 = call0;
+      b.tw = s->tw;
+      b.tws = (int)(s->nmax / s->sizes[k]);
+      if (frame < 0) b.res = nullptr;
+      HIPCHK(s->f64 ? gpe_run_sweep_f64(s->sizes[k], b, s->stream) : gpe_run_sweep_f32(s->sizes[k], b, s->stream));
+    }
+    return GPE_OK;
+  };
+  const uint64_t call0 = s->call;
+  double t = t0;
+  if (ts && save_start) ts[0] = t0;
+  HIPCHK(hipEventRecord(s->ev0, s->stream));
+  if (save_start && res) {
+    rc = launch_all(0, 0, 0, call0);  // writes u0 as frame 0
+    if (rc) return rc;
+  }
+  for (int64_t blk = 0; blk < nsaves; ++blk) {
+    rc = launch_all(sps, res ? blk + (save_start ? 1 : 0) : -1, 2 * blk * sps, call0 + 2 * (uint64_t)(blk * sps));
+    if (rc) return rc;
+    for (int64_t k = 0; k < sps; ++k) t += rdt;
+    if (ts) ts[blk + (save_start ? 1 : 0)] = t;
+  }
+  HIPCHK(hipEventRecord(s->ev1, s->stream));
+  if (res_host) HIPCHK(hipMemcpyAsync(result, s->d_res, res_bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+  s->call = call0 + 2 * (uint64_t)S;
+  return GPE_OK;
+}
+
+extern "C" int gpe_sweep_last_ms(gpe_sweep* s, float* ms) {
+  if (!s || !ms) return fail(GPE_ERR_INVALID, "gpe_sweep_last_ms: null argument");
+  *ms = s->last_ms;
   return GPE_OK;
 }

@@ -46,6 +46,18 @@ static hipError_t launch_1d(const Step1DArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename R, int N>
+static hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
+  constexpr int W = x_lines_per_wg(N);
+  constexpr int T = N / kE;
+  const size_t lds = xlds_bytes<R, N, 1, W>();
+  auto k = a.noise_kind ? sweep1d_kernel<R, N, W, true> : sweep1d_kernel<R, N, W, false>;
+  hipError_t e = set_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)((a.ninst + W - 1) / W)), dim3(W * T), lds, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace gpe
 
 #define GPE_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
@@ -60,6 +72,12 @@ static hipError_t launch_1d(const Step1DArgs& a, hipStream_t s) {
   hipError_t gpe_run_y_##SUFFIX(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s) {           \
     switch (N * 4 + M) {                                                                               \
       GPE_SIZES(GPE_YCASE_##SUFFIX)                                                                    \
+    }                                                                                                  \
+    return hipErrorInvalidValue;                                                                       \
+  }                                                                                                    \
+  hipError_t gpe_run_sweep_##SUFFIX(int64_t N, const gpe::SweepArgs& a, hipStream_t s) {               \
+    switch (N) {                                                                                       \
+      GPE_SIZES(GPE_SCASE_##SUFFIX)                                                                    \
     }                                                                                                  \
     return hipErrorInvalidValue;                                                                       \
   }                                                                                                    \

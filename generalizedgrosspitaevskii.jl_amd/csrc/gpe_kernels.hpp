@@ -70,14 +70,16 @@ __device__ __forceinline__ const void* selp(int c, const void* a0, const void* a
 // The Wiener increment -i sqrt(delta) eta_c xi_c of the thread's E values.
 // Element m and m + E/2 (x and x + N/2 on the line) share one Philox draw:
 // ctr = (P mod 2^32, c + 16 (P >> 32), call mod 2^32, call >> 32) with
-// P = line * N/2 + x, key = seed; words (x,y) feed x, (z,w) feed x + N/2.
+// P = pbase + x, key = seed; words (x,y) feed x, (z,w) feed x + N/2.
+// pbase is the flat index of the line's first point over 2 (line * N/2 for
+// equal lines; offset/2 for concatenated instances of uneven length).
 template <typename R, int N>
 __device__ __forceinline__ void noise_terms(Cx<R> (&nz)[kE], const HalfStep& hs, const PointTerms& pt, int c, R eta,
-                                            int t, int T, int64_t line) {
+                                            int t, int T, int64_t pbase) {
   const R a = (R)sqrt(hs.delta) * eta;
 #pragma unroll
   for (int m = 0; m < kE / 2; ++m) {
-    const uint64_t P = (uint64_t)line * (uint64_t)(N / 2) + (uint64_t)(t + m * T);
+    const uint64_t P = (uint64_t)pbase + (uint64_t)(t + m * T);
     U4 ctr;
     ctr.x = (uint32_t)P;
     ctr.y = (uint32_t)c + 16u * (uint32_t)(P >> 32);
@@ -140,16 +142,16 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const Poi
 
 // Real-space half-step (muladd_kernel!, src/kernels.jl:37-54) on the E values
 // of component c a thread holds; element m sits at dim-0 index x = t + m*T of
-// global line `line` (the Philox layout's line index).  When G couples the
+// a line whose first Philox pair index is `pbase`.  When G couples the
 // components, |u_j|^2 of the other components comes through the LDS slice
 // `a2` (M rows of N values for this thread's line); every thread of the
 // workgroup takes the same branch, so the barriers are uniform.
 template <typename R, int N, int M, bool NOISE>
 __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep& hs, const PointTerms& pt,
-                                          const PointCache<R, M>& pc, int t, int T, int64_t line, R* a2) {
+                                          const PointCache<R, M>& pc, int t, int T, int64_t pbase, R* a2) {
   const R delta = (R)hs.delta;
   Cx<R> nz[kE];
-  if constexpr (NOISE) noise_terms<R, N>(nz, hs, pt, c, pc.eta, t, T, line);
+  if constexpr (NOISE) noise_terms<R, N>(nz, hs, pt, c, pc.eta, t, T, pbase);
   R a2j[M][kE];
   if (pt.gmask) {
 #pragma unroll
@@ -342,14 +344,14 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
     fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
     __syncthreads();
   }
-  if (a.flags & kEnd) half_step<R, N, M, NOISE>(v, c, a.h_end, a.pt, pc, t, T, line, a2);
+  if (a.flags & kEnd) half_step<R, N, M, NOISE>(v, c, a.h_end, a.pt, pc, t, T, line * (N / 2), a2);
   void* fr = c ? a.frame[M - 1] : a.frame[0];
   if (fr != nullptr && valid) {
     Cx<R>* dst = reinterpret_cast<Cx<R>*>(fr) + line * N;
 #pragma unroll
     for (int m = 0; m < kE; ++m) dst[t + m * T] = v[m];
   }
-  if (a.flags & kStart) half_step<R, N, M, NOISE>(v, c, a.h_start, a.pt, pc, t, T, line, a2);
+  if (a.flags & kStart) half_step<R, N, M, NOISE>(v, c, a.h_start, a.pt, pc, t, T, line * (N / 2), a2);
   if (a.flags & kOutK) fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
   if (valid) {
 #pragma unroll
@@ -474,7 +476,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
     h.fnext[0] = q * en[1][0];
     h.fnext[1] = q * en[1][1];
     h.call = a.call0 + 2 * (uint64_t)s;
-    half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line, a2);
+    half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line * (N / 2), a2);
     fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
     apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
     __syncthreads();
@@ -484,7 +486,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
     h.fnext[0] = q * en[2][0];
     h.fnext[1] = q * en[2][1];
     h.call += 1;
-    half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line, a2);
+    half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line * (N / 2), a2);
     __syncthreads();
   }
   if (valid) {
@@ -625,6 +627,128 @@ __global__ void ev_table_kernel(EvArgs a) {
     const Cx<double> z = (a.form == 1) ? V[k] : V[k * a.M + c];
     const Cx<double> e = cexp_d(mk<double>(a.tau * z.im, -a.tau * z.re));
     reinterpret_cast<Cx<R>*>(a.out[c])[k] = mk<R>((R)e.re, (R)e.im);
+  }
+}
+
+// ------------------------------------------- uneven-size instance sweeps
+// A sweep holds many independent 1D single-component instances of different
+// lengths and parameters, concatenated in one buffer (the reference solves
+// them one GrossPitaevskiiProblem at a time).  One launch per distinct length
+// N processes that length's instances for a whole save block, each line
+// resident in registers like step1d_kernel.
+struct SweepInst {
+  int64_t off;       // first point in the concatenated field / tables
+  int64_t pump_off;  // first point of the pump profile, -1 if unpumped
+  int64_t res_off;   // first value of this instance's result block (frames of N values)
+  double g[2];       // G = c + g |u|^2 (complex)
+  double c[2];
+  double eta;        // constant position-noise amplitude
+};
+
+struct SweepArgs {
+  void* u;                   // concatenated field
+  void* res;                 // instance-major frames, or null
+  int64_t frame;             // frame index written at the end of the launch
+  const SweepInst* inst;     // all instances
+  const int32_t* ids;        // instances of this launch's length
+  int64_t ninst;             // number of ids
+  int64_t nsteps;
+  const double* env;         // shared complex envelope samples or null (== 1)
+  int64_t env_base;
+  double delta;              // dt/2
+  uint64_t call0;
+  const void* ed;            // exp(-i dt D_i(k)) per instance (scalar form, unscaled)
+  const void* pump;          // concatenated pump profiles
+  const void* tw;
+  int tws;
+  int noise_kind;
+  uint32_t seed_lo, seed_hi;
+  int g_cplx;
+};
+
+template <typename R, int N, int W, bool NOISE>
+__global__ void __launch_bounds__(W * (N / kE)) sweep1d_kernel(SweepArgs a) {
+  constexpr int T = N / kE;
+  constexpr int LS = lds_line_stride(N);
+  extern __shared__ unsigned char smem_raw[];
+  Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int l = tid / T, t = tid % T;
+  const int64_t j_raw = (int64_t)blockIdx.x * W + l;
+  const bool valid = j_raw < a.ninst;
+  const SweepInst in = a.inst[a.ids[valid ? j_raw : a.ninst - 1]];
+  Cx<R>* uc = reinterpret_cast<Cx<R>*>(a.u) + in.off;
+  Cx<R> v[kE];
+#pragma unroll
+  for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
+  TwReg<R, N> tr;
+  load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
+  // per-instance terms: exp(-i dt D)/N, pump profile, G and eta of row 0
+  const R invn = (R)1 / (R)N;
+  EdVals<R> ev;
+#pragma unroll
+  for (int m = 0; m < kE; ++m) ev.own[m] = scale(ldc<R>(a.ed, in.off + t + m * T), invn);
+  PointTerms pt;
+  memset(&pt, 0, sizeof pt);
+  pt.gmask = (in.g[0] != 0.0 || in.g[1] != 0.0 || in.c[0] != 0.0 || in.c[1] != 0.0) ? 1 : 0;
+  pt.g_cplx = a.g_cplx;
+  pt.g[0][0][0] = in.g[0];
+  pt.g[0][0][1] = in.g[1];
+  pt.c[0][0] = in.c[0];
+  pt.c[0][1] = in.c[1];
+  pt.noise_kind = a.noise_kind;
+  pt.eta[0] = in.eta;
+  pt.seed_lo = a.seed_lo;
+  pt.seed_hi = a.seed_hi;
+  pt.pump_kind = (in.pump_off >= 0 && a.pump) ? 2 : 0;
+  pt.pump_a[0] = pt.pump_kind ? (const void*)(reinterpret_cast<const Cx<R>*>(a.pump) + in.pump_off) : nullptr;
+  PointCache<R, 1> pc;
+  load_point_cache<R, 1>(pc, pt, 0, t, T, 0, 0);
+  Cx<R>* my = lds + l * LS;
+  const int boff = W * LS;
+  EdDesc ed;
+  memset(&ed, 0, sizeof ed);
+  ed.form = 1;
+  const double q = 0.5 * a.delta;
+  const int64_t pbase = in.off / 2;
+  for (int64_t s = 0; s < a.nsteps; ++s) {
+    HalfStep h;
+    h.delta = a.delta;
+    const int64_t e0 = a.env_base + 2 * s;
+    double en[3][2] = {{1, 0}, {1, 0}, {1, 0}};
+    if (a.env) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        en[i][0] = a.env[2 * (e0 + i)];
+        en[i][1] = a.env[2 * (e0 + i) + 1];
+      }
+    }
+    h.fnow[0] = q * en[0][0];
+    h.fnow[1] = q * en[0][1];
+    h.fnext[0] = q * en[1][0];
+    h.fnext[1] = q * en[1][1];
+    h.call = a.call0 + 2 * (uint64_t)s;
+    half_step<R, N, 1, NOISE>(v, 0, h, pt, pc, t, T, pbase, nullptr);
+    fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
+    apply_ed<R, 1>(v, ed, ev, t, T, my, my);
+    __syncthreads();
+    fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
+    h.fnow[0] = q * en[1][0];
+    h.fnow[1] = q * en[1][1];
+    h.fnext[0] = q * en[2][0];
+    h.fnext[1] = q * en[2][1];
+    h.call += 1;
+    half_step<R, N, 1, NOISE>(v, 0, h, pt, pc, t, T, pbase, nullptr);
+    __syncthreads();
+  }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) uc[t + m * T] = v[m];
+    if (a.res) {
+      Cx<R>* dst = reinterpret_cast<Cx<R>*>(a.res) + in.res_off + a.frame * N;
+#pragma unroll
+      for (int m = 0; m < kE; ++m) dst[t + m * T] = v[m];
+    }
   }
 }
 

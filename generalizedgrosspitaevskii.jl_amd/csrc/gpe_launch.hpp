@@ -21,5 +21,7 @@ hipError_t gpe_run_x_f32(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlin
 hipError_t gpe_run_x_f64(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, hipStream_t s);
 hipError_t gpe_run_y_f32(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s);
 hipError_t gpe_run_y_f64(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s);
+hipError_t gpe_run_sweep_f32(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
+hipError_t gpe_run_sweep_f64(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
 hipError_t gpe_run_1d_f32(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);
 hipError_t gpe_run_1d_f64(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);

@@ -328,7 +328,20 @@ def pump_descriptor(prob, r_axes, spatial, times):
             v = arr[c_star, 0]
         return complex(v)
 
-    env = np.array([env_point(t) for t in times], dtype=np.complex128) / p_star
+    env = None
+    try:  # closures written with numpy broadcast over an array of times
+        kind, arr = _form(f(r_star, prob.param, np.asarray(times)), (nt,))
+        arr = arr if kind == "S" else (arr[c_star] if kind == "V" else arr[c_star, 0])
+        env = np.array(arr, dtype=np.complex128) / p_star
+        # spot-check the vectorised evaluation against scalar calls
+        for i in (0, nt // 2, nt - 1):
+            if env[i] != env_point(times[i]) / p_star:
+                env = None
+                break
+    except Exception:
+        env = None
+    if env is None:
+        env = np.array([env_point(t) for t in times], dtype=np.complex128) / p_star
     # verify separability on the probe times and a few more
     rng = np.random.default_rng(0xF00)
     check = set(probe_idx) | set(rng.integers(0, nt, size=min(nt, 8)).tolist())
@@ -601,5 +614,156 @@ def solve(prob, alg, tspan, *, dt, nsaves, show_progress=False, progress=None, s
     it = init(prob, alg, tspan, dt=dt, nsaves=nsaves, save_start=save_start, rng=rng)
     try:
         return it.solve(save_frames=save_frames, ensemble_moments=ensemble_moments)
+    finally:
+        it.close()
+
+
+# --------------------------------------------------------------- sweeps
+class SweepIterator:
+    """Many independent 1D single-component problems of uneven grid size and
+    parameters advanced together (gpe_sweep_* of include/gpe_hip.h).  The
+    reference runs such a sweep as one solve per GrossPitaevskiiProblem."""
+
+    def __init__(self, problems, tspan, dt, nsaves, save_start=True, rng=None):
+        lib = L.load()
+        problems = list(problems)
+        if not problems:
+            raise ValueError("empty sweep")
+        for p in problems:
+            if p.N != 1 or p.M != 1 or len(p.u0[0].shape) != 1:
+                raise NotImplementedError("sweeps take 1D single-component problems without batch dims")
+        dtypes = {str(p.u0[0].dtype).replace("torch.", "") for p in problems}
+        if len(dtypes) != 1:
+            raise TypeError("all instances of a sweep need the same field precision")
+        self.problems = problems
+        self.tspan = (float(tspan[0]), float(tspan[1]))
+        self.dt_in = float(dt)
+        self.nsaves = int(nsaves)
+        self.save_start = bool(save_start)
+        self.dt, self.steps_per_save = resolve_fixed_timestepping(dt, tspan, nsaves)
+        self.rng = _rng_of(rng)
+        self.n = np.array([p.u0[0].shape[0] for p in problems], dtype=np.int64)
+        self.off = np.concatenate([[0], np.cumsum(self.n)[:-1]]).astype(np.int64)
+        self.total = int(self.n.sum())
+        Ls = np.array([p.lengths[0] for p in problems], dtype=np.float64)
+        self.dtype = dtypes.pop()
+        h = ctypes.c_void_p()
+        L.check(lib.gpe_sweep_create(len(problems), self.n.ctypes.data, Ls.ctypes.data,
+                                     L.GPE_C64 if self.dtype == "complex64" else L.GPE_C128, 0, ctypes.byref(h)))
+        self.h, self._lib = h, lib
+        try:
+            self._set_terms()
+        except BaseException:
+            lib.gpe_sweep_destroy(h)
+            self.h = None
+            raise
+
+    def _set_terms(self):
+        lib, h, probs = self._lib, self.h, self.problems
+        D = np.zeros(self.total, dtype=np.complex128)
+        g = np.zeros(len(probs), dtype=np.complex128)
+        c = np.zeros(len(probs), dtype=np.complex128)
+        for i, p in enumerate(probs):
+            sl = slice(self.off[i], self.off[i] + self.n[i])
+            if p.dispersion is not None:
+                kind, arr = _form(p.dispersion(reciprocal_grid(p), p.param), (self.n[i],))
+                arr = arr if kind == "S" else (arr[0] if kind == "V" else arr[0, 0])
+                D[sl] = arr
+            if p.potential is not None:
+                raise NotImplementedError("sweeps: potentials are not supported")
+            if p.nonlinearity is not None:
+                gi, ci = fit_nonlinearity(p.nonlinearity, 1, p.param)
+                g[i], c[i] = gi[0, 0], ci[0]
+        self._keep = [D, g, c]
+        L.check(lib.gpe_sweep_set_dispersion(h, D.ctypes.data))
+        L.check(lib.gpe_sweep_set_nonlinearity(h, g.ctypes.data, c.ctypes.data))
+        if any(p.pump is not None for p in probs):
+            times = pump_times(self.tspan, self.dt_in, self.nsaves)
+            prof = np.zeros(self.total, dtype=np.complex128)
+            envs = []
+            for i, p in enumerate(probs):
+                if p.pump is None:
+                    envs.append(None)
+                    continue
+                kind, pr, env = pump_descriptor(p, (direct_grid(p)[0],), (self.n[i],), times)
+                if kind == L.GPE_PUMP_NONE:
+                    envs.append(None)
+                    continue
+                prof[self.off[i]:self.off[i] + self.n[i]] = pr.view(np.complex128)
+                envs.append(None if env is None else env.view(np.complex128))
+            ref = next((e for e in envs if e is not None), None)
+            if ref is not None:
+                j = int(np.argmax(np.abs(ref)))
+                for i, e in enumerate(envs):
+                    if probs[i].pump is None:
+                        continue
+                    e = np.ones_like(ref) if e is None else e
+                    a = e[j] / ref[j]
+                    if np.max(np.abs(e - a * ref)) > 1e-12 * np.max(np.abs(e)):
+                        raise NotImplementedError("sweeps share one pump envelope up to a factor per instance")
+                    prof[self.off[i]:self.off[i] + self.n[i]] *= a
+            envf = None if ref is None else np.ascontiguousarray(ref).view(np.float64)
+            proff = np.ascontiguousarray(prof).view(np.float64)
+            self._keep += [proff, envf]
+            L.check(lib.gpe_sweep_set_pump(h, proff.ctypes.data, None if envf is None else envf.ctypes.data,
+                                           0 if envf is None else len(envf) // 2))
+        noisy = [p.noise_prototype is not None for p in probs]
+        if any(noisy):
+            if not all(noisy):
+                raise NotImplementedError("sweeps: give every instance a noise_prototype, or none")
+            eta = np.zeros(len(probs))
+            for i, p in enumerate(probs):
+                if p.position_noise_func is None:
+                    raise TypeError("noise_prototype without position_noise_func (no method in the reference)")
+                eta[i] = noise_eta(p.position_noise_func, 1, p.param, direct_grid(p))[0]
+            first = probs[0].noise_prototype
+            first = first[0] if isinstance(first, (tuple, list)) else first
+            cplx = np.iscomplexobj(np.empty(0, dtype=str(first.dtype).replace("torch.", "")))
+            self._keep.append(eta)
+            L.check(lib.gpe_sweep_set_noise(h, L.GPE_NOISE_COMPLEX if cplx else L.GPE_NOISE_REAL, eta.ctypes.data,
+                                            self.rng.seed, self.rng.counter))
+
+    def solve(self, save_frames=True):
+        """Returns (ts, results): results[i] has shape (n_i, nsaves + save_start)."""
+        lib = self._lib
+        nfr = self.nsaves + int(self.save_start)
+        u0 = np.concatenate([np.asarray(p.u0[0]) for p in self.problems]).astype(self.dtype)
+        res = np.empty(self.total * nfr, dtype=self.dtype) if save_frames else None
+        ts = np.empty(nfr)
+        L.check(lib.gpe_sweep_solve(self.h, u0.ctypes.data, self.tspan[0], self.tspan[1], self.dt_in, self.nsaves,
+                                    int(self.save_start), None if res is None else res.ctypes.data, ts.ctypes.data))
+        if any(p.noise_prototype is not None for p in self.problems):
+            self.rng.counter += 2 * self.steps_per_save * self.nsaves
+        if res is None:
+            return ts, None
+        out = [res[self.off[i] * nfr:(self.off[i] + self.n[i]) * nfr].reshape((self.n[i], nfr), order="F")
+               for i in range(len(self.problems))]
+        return ts, out
+
+    def last_ms(self):
+        ms = ctypes.c_float()
+        L.check(self._lib.gpe_sweep_last_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.gpe_sweep_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_sweep(problems, alg, tspan, *, dt, nsaves, save_start=True, rng=None, save_frames=True):
+    """Solve many 1D problems of uneven size in one go: the equivalent of
+    [solve(p, alg, tspan; dt, nsaves) for p in problems] (ts is shared)."""
+    if not isinstance(alg, StrangSplitting):
+        raise TypeError("only StrangSplitting() is implemented")
+    it = SweepIterator(problems, tspan, dt, nsaves, save_start, rng)
+    try:
+        return it.solve(save_frames=save_frames)
     finally:
         it.close()

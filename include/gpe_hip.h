@@ -160,6 +160,48 @@ int gpe_advance(gpe_handle* h, int64_t nsteps, float* ms);
  * the same stream, the bracketing's own latency. */
 int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y, double* ms_event);
 
+/* ----------------------------------------------------------------------
+ * Uneven-size instance sweeps.  A sweep is a set of independent 1D,
+ * single-component problems of different grid sizes n[i] (powers of two
+ * 16..4096) and lengths L[i], each with its own D(k), G and pump profile and
+ * a shared pump envelope, advanced together on one GPU.  It replaces the
+ * reference's way of running such a sweep, one solve per
+ * GrossPitaevskiiProblem (src/fixed_time_stepping.jl:79-81) in a loop.
+ * Concatenated layout: instance i occupies points off[i] .. off[i]+n[i]-1,
+ * off[i] = n[0] + ... + n[i-1].
+ * ---------------------------------------------------------------------- */
+typedef struct gpe_sweep gpe_sweep;
+
+int gpe_sweep_create(int64_t ninst, const int64_t* n, const double* L, int32_t dtype, int32_t device,
+                     gpe_sweep** out);
+int gpe_sweep_destroy(gpe_sweep* s);
+
+/* Concatenated D_i(k) on each instance's reciprocal grid (FFT order),
+ * scalar form, complex FP64 (sum n[i] values). */
+int gpe_sweep_set_dispersion(gpe_sweep* s, const double* D);
+
+/* Per-instance G_i(u) = c[i] + g[i] |u|^2 (complex FP64, ninst values each). */
+int gpe_sweep_set_nonlinearity(gpe_sweep* s, const double* g, const double* c);
+
+/* Concatenated pump profiles (complex FP64, sum n[i] values; an all-zero
+ * block leaves that instance unpumped) times one envelope shared by all
+ * instances, sampled at the times gpe_pump_times lists (n_env == 0: 1). */
+int gpe_sweep_set_pump(gpe_sweep* s, const double* profile, const double* envelope, int64_t n_env);
+
+/* Per-instance constant noise amplitude eta[i] with Philox key `seed`. */
+int gpe_sweep_set_noise(gpe_sweep* s, int32_t kind, const double* eta, uint64_t seed, uint64_t call0);
+
+/* The fixed-step solve of every instance.  u0: concatenated initial fields.
+ * result: instance-major frames, instance i's block of n[i] * nframes values
+ * (one column-major (n[i], nframes) array, the reference's result shape) at
+ * offset off[i] * nframes, nframes = nsaves + save_start; NULL skips frames.
+ * u0/result may be host or device memory; ts as in gpe_solve. */
+int gpe_sweep_solve(gpe_sweep* s, const void* u0, double t0, double t1, double dt, int64_t nsaves,
+                    int32_t save_start, void* result, double* ts);
+
+/* Device time in ms of the step loop of the last gpe_sweep_solve. */
+int gpe_sweep_last_ms(gpe_sweep* s, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

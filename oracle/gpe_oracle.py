@@ -259,12 +259,13 @@ def philox4x32_10(c0, c1, c2, c3, k0, k1):
     return (c0.astype(np.uint32), c1.astype(np.uint32), c2.astype(np.uint32), c3.astype(np.uint32))
 
 
-def philox_xi(shape, comp, seed, call, complex_noise=True):
+def philox_xi(shape, comp, seed, call, complex_noise=True, pair_base=0):
     """xi for one component of one half-step, FP64.
 
     Counter layout (shared with the HIP library): the flat column-major index
     p over ``shape`` splits into line l = p // N0 and x = p % N0 along the
-    first axis; pair index P = l*(N0/2) + (x mod N0/2) and half h = x // (N0/2).
+    first axis; pair index P = pair_base + l*(N0/2) + (x mod N0/2) and half
+    h = x // (N0/2) (pair_base: an instance's offset / 2 inside a sweep).
     ctr = (P mod 2^32, comp + 16*(P >> 32), call mod 2^32, call >> 32),
     key = (seed mod 2^32, seed >> 32).  Words (x0,x1) feed h=0, (x2,x3) h=1.
     Box-Muller: u = (w + 0.5) 2^-32, z = sqrt(-log u_a) e^{2 pi i u_b};
@@ -280,7 +281,7 @@ def philox_xi(shape, comp, seed, call, complex_noise=True):
     line = p // N0
     x = p % N0
     h = x // half
-    P = line * half + (x % half)
+    P = int(pair_base) + line * half + (x % half)
     call = int(call)
     w0, w1, w2, w3 = philox4x32_10(P & 0xFFFFFFFF, comp + 16 * (P >> 32), call & 0xFFFFFFFF,
                                    (call >> 32) & 0xFFFFFFFF, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
@@ -314,6 +315,7 @@ class Problem:
     param: Any = None
     noise_seed: int = 0
     noise_call0: int = 0
+    noise_pair_base: int = 0  # Philox pair index of the first point (an instance's offset / 2 in a sweep)
 
     def __post_init__(self):
         # problem.jl:112-115
@@ -393,7 +395,8 @@ class Iterator:
             return AI
         cplx = p.noise_prototype != "real"
         M = len(self.u)
-        xi = np.stack([philox_xi(self.shape, c, p.noise_seed, self.noise_call, cplx) for c in range(M)])
+        xi = np.stack([philox_xi(self.shape, c, p.noise_seed, self.noise_call, cplx, p.noise_pair_base)
+                       for c in range(M)])
         self.noise_call += 1
         return Form("V", xi)
 
