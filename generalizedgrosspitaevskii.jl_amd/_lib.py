@@ -84,6 +84,9 @@ SIGNATURES = {
     "gpe_last_solve_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
     "gpe_advance": (ctypes.c_int, [P, I64, ctypes.POINTER(ctypes.c_float)]),
     "gpe_profile_kernels": (ctypes.c_int, [P, I64, D, D, D]),
+    "gpe_steady_state": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_double, ctypes.c_double, I64, I64,
+                                         ctypes.POINTER(I64), D, P, I64, ctypes.POINTER(I64),
+                                         ctypes.POINTER(ctypes.c_float)]),
     "gpe_sweep_create": (ctypes.c_int, [I64, P, P, I32, I32, ctypes.POINTER(P)]),
     "gpe_sweep_destroy": (ctypes.c_int, [P]),
     "gpe_sweep_set_dispersion": (ctypes.c_int, [P, P]),

@@ -1248,3 +1248,157 @@ extern "C" int gpe_sweep_last_ms(gpe_sweep* s, float* ms) {
   *ms = s->last_ms;
   return GPE_OK;
 }
+
+// ------------------------------------------------------------ steady state
+// Step from u0 until r = ||u_{n+1} - u_n|| / ||u_{n+1}|| <= tol, checking
+// every `check_every` steps on two consecutive real-space fields.
+struct SteadyBufs {
+  void* a[kMaxM] = {nullptr, nullptr};
+  void* b[kMaxM] = {nullptr, nullptr};
+  double* partial = nullptr;
+  double* result = nullptr;
+  double* host = nullptr;  // pinned pair (diff^2, norm^2)
+};
+
+static void free_steady(SteadyBufs& sb) {
+  for (int c = 0; c < kMaxM; ++c) {
+    dfree(sb.a[c]);
+    dfree(sb.b[c]);
+  }
+  void* p = sb.partial;
+  dfree(p);
+  p = sb.result;
+  dfree(p);
+  if (sb.host) (void)hipHostFree(sb.host);
+  sb.host = nullptr;
+}
+
+extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt, double tol, int64_t check_every,
+                                int64_t max_steps, int64_t* steps, double* residual, double* history,
+                                int64_t max_hist, int64_t* nhist, float* ms) {
+  if (!h || !u0 || !steps || !residual) return fail(GPE_ERR_INVALID, "gpe_steady_state: null argument");
+  if (!(dt > 0) || !(tol > 0) || check_every < 2 || max_steps < check_every)
+    return fail(GPE_ERR_INVALID, "gpe_steady_state: dt=%g tol=%g check_every=%lld max_steps=%lld", dt, tol,
+                (long long)check_every, (long long)max_steps);
+  if (!h->env.empty()) return fail(GPE_ERR_STATE, "gpe_steady_state: the pump envelope must be static");
+  HIPCHK(hipSetDevice(h->d.device));
+  int rc = prepare_tables(h, dt);
+  if (rc) return rc;
+  const size_t fb = field_bytes(h);
+  SteadyBufs sb;
+  const int nblk = 512;
+  auto cleanup = [&](int code) {
+    free_steady(sb);
+    return code;
+  };
+  for (int c = 0; c < h->M; ++c)
+    if (hipMalloc(&sb.a[c], fb) != hipSuccess || hipMalloc(&sb.b[c], fb) != hipSuccess)
+      return cleanup(fail(GPE_ERR_NOMEM, "gpe_steady_state: field buffers"));
+  if (hipMalloc((void**)&sb.partial, 4 * nblk * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&sb.result, 2 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void**)&sb.host, 2 * sizeof(double)) != hipSuccess)
+    return cleanup(fail(GPE_ERR_NOMEM, "gpe_steady_state: reduction buffers"));
+  for (int c = 0; c < h->M; ++c)
+    if (hipMemcpyAsync(h->u[c], u0[c], fb, hipMemcpyDefault, h->stream) != hipSuccess)
+      return cleanup(fail(GPE_ERR_DEVICE, "gpe_steady_state: upload"));
+  auto residual_of = [&](double* r) -> int {
+    ResidualArgs ra;
+    memset(&ra, 0, sizeof ra);
+    for (int c = 0; c < h->M; ++c) {
+      ra.a[c] = sb.b[c];
+      ra.b[c] = sb.a[c];
+    }
+    ra.npts = h->npts;
+    ra.M = h->M;
+    ra.partial = sb.partial;
+    if (h->f64)
+      hipLaunchKernelGGL(residual_partial_kernel<double>, dim3(nblk), dim3(256), 0, h->stream, ra);
+    else
+      hipLaunchKernelGGL(residual_partial_kernel<float>, dim3(nblk), dim3(256), 0, h->stream, ra);
+    hipLaunchKernelGGL(residual_final_kernel<0>, dim3(1), dim3(64), 0, h->stream, (const double*)sb.partial, nblk,
+                       sb.result);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sb.host, sb.result, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *r = sb.host[1] > 0 ? std::sqrt(sb.host[0] / sb.host[1]) : (sb.host[0] > 0 ? INFINITY : 0.0);
+    return GPE_OK;
+  };
+  const uint64_t call0 = h->call;
+  int64_t s = 0, nh = 0;
+  double r = INFINITY;
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  if (h->ndim == 2) {
+    XPassArgs xa = x_args(h);
+    const YPassArgs ya = y_args(h);
+    xa.flags = kStart | kOutK;
+    xa.h_start = make_half(h, dt, 1, 0, call0);
+    rc = run_x(h, xa) == hipSuccess ? GPE_OK : fail(GPE_ERR_DEVICE, "gpe_steady_state: launch");
+    while (!rc) {
+      ++s;
+      if (run_y(h, ya) != hipSuccess) {
+        rc = fail(GPE_ERR_DEVICE, "gpe_steady_state: launch");
+        break;
+      }
+      const bool prev = (s % check_every) == check_every - 1;
+      const bool check = (s % check_every) == 0;
+      xa.h_end = make_half(h, dt, s, 1, call0);
+      xa.h_start = make_half(h, dt, s + 1, 0, call0);
+      xa.flags = kInK | kEnd | kStart | kOutK;
+      for (int c = 0; c < kMaxM; ++c) xa.frame[c] = c < h->M ? (prev ? sb.a[c] : (check ? sb.b[c] : nullptr)) : nullptr;
+      if (run_x(h, xa) != hipSuccess) {
+        rc = fail(GPE_ERR_DEVICE, "gpe_steady_state: launch");
+        break;
+      }
+      if (check) {
+        rc = residual_of(&r);
+        if (rc) break;
+        if (history && nh < max_hist) history[nh] = r;
+        ++nh;
+        if (r <= tol || s + check_every > max_steps) break;
+      }
+    }
+    // the real-space field after the last checked step becomes the state
+    if (!rc)
+      for (int c = 0; c < h->M; ++c)
+        if (hipMemcpyAsync(h->u[c], sb.b[c], fb, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+          rc = fail(GPE_ERR_DEVICE, "gpe_steady_state: copy");
+  } else {
+    Step1DArgs a = s1_args(h, dt);
+    a.env = nullptr;
+    a.env_base = 0;
+    while (!rc) {
+      a.nsteps = check_every - 1;
+      a.call0 = call0 + 2 * (uint64_t)s;
+      for (int c = 0; c < kMaxM; ++c) a.frame[c] = c < h->M ? sb.a[c] : nullptr;
+      if (run_1d(h, a) != hipSuccess) {
+        rc = fail(GPE_ERR_DEVICE, "gpe_steady_state: launch");
+        break;
+      }
+      a.nsteps = 1;
+      a.call0 = call0 + 2 * (uint64_t)(s + check_every - 1);
+      for (int c = 0; c < kMaxM; ++c) a.frame[c] = c < h->M ? sb.b[c] : nullptr;
+      if (run_1d(h, a) != hipSuccess) {
+        rc = fail(GPE_ERR_DEVICE, "gpe_steady_state: launch");
+        break;
+      }
+      s += check_every;
+      rc = residual_of(&r);
+      if (rc) break;
+      if (history && nh < max_hist) history[nh] = r;
+      ++nh;
+      if (r <= tol || s + check_every > max_steps) break;
+    }
+  }
+  if (rc) return cleanup(rc);
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  h->call = call0 + 2 * (uint64_t)s;
+  h->configured = true;
+  h->dt = dt;
+  *steps = s;
+  *residual = r;
+  if (nhist) *nhist = nh;
+  if (ms) *ms = h->last_ms;
+  return cleanup(GPE_OK);
+}

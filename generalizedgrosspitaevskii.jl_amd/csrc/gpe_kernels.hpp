@@ -803,4 +803,65 @@ __global__ void observables_kernel(ObsArgs a) {
   }
 }
 
+// ------------------------------------------------ steady-state residual
+// sum |a - b|^2 and sum |a|^2 over all points and components in FP64 with
+// Neumaier compensation: per-block partials, then one block folds them.
+struct ResidualArgs {
+  const void* a[kMaxM];  // u_{n+1} (real space)
+  const void* b[kMaxM];  // u_n
+  int64_t npts;
+  int M;
+  double* partial;       // 4 doubles per block: diff sum/comp, norm sum/comp
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) residual_partial_kernel(ResidualArgs r) {
+  __shared__ double sh[4][256];
+  double sd = 0, cd = 0, sn = 0, cn = 0;
+  for (int c = 0; c < r.M; ++c) {
+    const Cx<R>* a = reinterpret_cast<const Cx<R>*>(r.a[c]);
+    const Cx<R>* b = reinterpret_cast<const Cx<R>*>(r.b[c]);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r.npts;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      const Cx<R> x = a[i], y = b[i];
+      const double dr = (double)x.re - (double)y.re, di = (double)x.im - (double)y.im;
+      neumaier_add(sd, cd, dr * dr + di * di);
+      neumaier_add(sn, cn, (double)x.re * (double)x.re + (double)x.im * (double)x.im);
+    }
+  }
+  sh[0][threadIdx.x] = sd;
+  sh[1][threadIdx.x] = cd;
+  sh[2][threadIdx.x] = sn;
+  sh[3][threadIdx.x] = cn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) {
+      neumaier_add(s0, c0, sh[0][i]);
+      c0 += sh[1][i];
+      neumaier_add(s1, c1, sh[2][i]);
+      c1 += sh[3][i];
+    }
+    double* o = r.partial + 4 * blockIdx.x;
+    o[0] = s0;
+    o[1] = c0;
+    o[2] = s1;
+    o[3] = c1;
+  }
+}
+
+template <int UNUSED = 0>
+__global__ void residual_final_kernel(const double* partial, int nblocks, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
+  for (int i = 0; i < nblocks; ++i) {
+    neumaier_add(s0, c0, partial[4 * i]);
+    c0 += partial[4 * i + 1];
+    neumaier_add(s1, c1, partial[4 * i + 2]);
+    c1 += partial[4 * i + 3];
+  }
+  out[0] = s0 + c0;
+  out[1] = s1 + c1;
+}
+
 }  // namespace gpe

@@ -587,6 +587,47 @@ class StrangSplittingIterator:
         L.check(self._lib.gpe_last_solve_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
+    def get_field(self):
+        """The handle's current real-space field (host numpy arrays)."""
+        out = tuple(np.empty(self.shape, dtype=str(x.dtype).replace("torch.", ""), order="F") for x in self.prob.u0)
+        L.check(self._lib.gpe_get_field(self.h, L.ptr_array([x.ctypes.data for x in out])))
+        return out
+
+    def steady_state(self, u0=None, tol=1e-8, check_every=50, max_steps=200000):
+        """Step with this iterator's resolved dt from u0 (default prob.u0) until
+        ||u_{n+1} - u_n|| / ||u_{n+1}|| <= tol (FP64, every check_every steps).
+        Returns (u, info) with info = dict(steps, residual, history, ms)."""
+        lib = self._lib
+        src = self.prob.u0 if u0 is None else tuple(u0)
+        if self.on_device:
+            u = [x if _contig_colmajor(x) else _to_colmajor(x) for x in src]
+            ptrs = [x.data_ptr() for x in u]
+        else:
+            u = [np.asfortranarray(np.asarray(x).astype(str(self.prob.u0[0].dtype))) for x in src]
+            ptrs = [x.ctypes.data for x in u]
+        steps = ctypes.c_int64()
+        res = ctypes.c_double()
+        nh = ctypes.c_int64()
+        ms = ctypes.c_float()
+        maxh = max_steps // check_every + 1
+        hist = np.zeros(maxh)
+        L.check(lib.gpe_steady_state(self.h, L.ptr_array(ptrs), self.dt, float(tol), int(check_every), int(max_steps),
+                                     ctypes.byref(steps), ctypes.byref(res), hist.ctypes.data, maxh, ctypes.byref(nh),
+                                     ctypes.byref(ms)))
+        return self.get_field(), dict(steps=steps.value, residual=res.value, history=hist[:min(nh.value, maxh)],
+                                      ms=ms.value)
+
+
+def steady_state(prob, alg, *, dt, tol=1e-8, check_every=50, max_steps=200000, u0=None):
+    """Run prob (static pump) from u0 (default prob.u0) until the relative
+    step-to-step change ||u_{n+1} - u_n|| / ||u_{n+1}|| <= tol.  No reference
+    counterpart: the reference integrates a fixed tspan.  Returns (u, info)."""
+    it = init(prob, alg, (0.0, float(dt)), dt=dt, nsaves=1)
+    try:
+        return it.steady_state(u0=u0, tol=tol, check_every=check_every, max_steps=max_steps)
+    finally:
+        it.close()
+
 
 def _contig_colmajor(x):
     return tuple(x.stride()) == _colmajor_strides(tuple(x.shape))

@@ -160,6 +160,20 @@ int gpe_advance(gpe_handle* h, int64_t nsteps, float* ms);
  * the same stream, the bracketing's own latency. */
 int gpe_profile_kernels(gpe_handle* h, int64_t nsteps, double* ms_x, double* ms_y, double* ms_event);
 
+/* Steady-state search.  The reference has no convergence check (users solve
+ * over a long tspan, src/fixed_time_stepping.jl:26-54); this steps from u0
+ * with the given dt (static pump envelope required) and every check_every
+ * steps (>= 2) forms r = ||u_{n+1} - u_n||_2 / ||u_{n+1}||_2 over all points
+ * and components in FP64 with compensated sums, from two consecutive
+ * real-space fields.  It stops at r <= tol or when max_steps would be
+ * exceeded.  *steps and *residual receive the steps taken and the last r;
+ * history (may be NULL) receives r of every check (at most max_hist, count in
+ * *nhist); *ms (may be NULL) the device time.  The final real-space field
+ * stays in the handle (gpe_get_field). */
+int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt, double tol, int64_t check_every,
+                     int64_t max_steps, int64_t* steps, double* residual, double* history, int64_t max_hist,
+                     int64_t* nhist, float* ms);
+
 /* ----------------------------------------------------------------------
  * Uneven-size instance sweeps.  A sweep is a set of independent 1D,
  * single-component problems of different grid sizes n[i] (powers of two
