@@ -87,6 +87,12 @@ SIGNATURES = {
     "gpe_steady_state": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_double, ctypes.c_double, I64, I64,
                                          ctypes.POINTER(I64), D, P, I64, ctypes.POINTER(I64),
                                          ctypes.POINTER(ctypes.c_float)]),
+    "gpe_fno_gemm": (ctypes.c_int, [P, P, P, I64, I64, I64, P, I64, ctypes.c_float, ctypes.c_float, P]),
+    "gpe_fno_bias_act": (ctypes.c_int, [P, P, P, P, I64, I64, I64, I32, I32, P]),
+    "gpe_fno_act_backward": (ctypes.c_int, [P, P, P, I64, P]),
+    "gpe_fno_axpby": (ctypes.c_int, [P, P, ctypes.c_float, ctypes.c_float, I64, P]),
+    "gpe_fno_adam": (ctypes.c_int, [P, P, P, P, I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     I64, P]),
     "gpe_sweep_create": (ctypes.c_int, [I64, P, P, I32, I32, ctypes.POINTER(P)]),
     "gpe_sweep_destroy": (ctypes.c_int, [P]),
     "gpe_sweep_set_dispersion": (ctypes.c_int, [P, P]),

@@ -31,7 +31,7 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 
-UNITS = ["gpe_inst_f32.hip", "gpe_inst_f64.hip", "gpe_api.hip"]
+UNITS = ["gpe_inst_f32.hip", "gpe_inst_f64.hip", "gpe_api.hip", "fno_kernels.hip"]
 
 
 def _compile(unit: str) -> str:

@@ -216,6 +216,37 @@ int gpe_sweep_solve(gpe_sweep* s, const void* u0, double t0, double t1, double d
 /* Device time in ms of the step loop of the last gpe_sweep_solve. */
 int gpe_sweep_last_ms(gpe_sweep* s, float* ms);
 
+/* ----------------------------------------------------------------------
+ * Kernels of the Fourier-neural-operator warm start for steady states.  The
+ * reference has no learned component; these serve the surrogate whose output
+ * seeds gpe_steady_state.  All arrays are device FP32, row-major; `stream` is
+ * a hipStream_t (NULL: the default stream).
+ * ---------------------------------------------------------------------- */
+
+/* C[b] = alpha A[b] B[b] + beta C[b] for b < batch (<= 65535): A is m x k,
+ * B k x n, C m x n, element (i, j) of each at base + i*rs + j*cs and batch b
+ * at + b*bs, with strides = {A rs, A cs, B rs, B cs, C rs, C cs, A bs, B bs,
+ * C bs} in elements (transposes are stride swaps).  Operands are rounded to
+ * BF16 and multiplied on the matrix cores with FP32 accumulation. */
+int gpe_fno_gemm(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k,
+                 const int64_t* strides, int64_t batch, float alpha, float beta, void* stream);
+
+/* y = act(x + bias[ch]) over rows x cols values, ch = column (channels_last)
+ * or (row mod C); act 1 = GELU (tanh form), 0 = identity; pre (may be NULL)
+ * keeps x + bias for the backward. */
+int gpe_fno_bias_act(const float* x, const float* bias, float* pre, float* y, int64_t rows, int64_t cols,
+                     int64_t C, int32_t channels_last, int32_t act, void* stream);
+
+/* dx = dy * GELU'(pre). */
+int gpe_fno_act_backward(const float* pre, const float* dy, float* dx, int64_t n, void* stream);
+
+/* y = a x + b y over n values. */
+int gpe_fno_axpby(const float* x, float* y, float a, float b, int64_t n, void* stream);
+
+/* One Adam step (bias-corrected with `step` >= 1) on n FP32 parameters. */
+int gpe_fno_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                 float eps, int64_t step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
