@@ -215,6 +215,231 @@ def sweep_run(g, torch, ninst=4096, steps=4000, dtype=np.complex64):
                 bound="VALU (each line stays in registers for 20 steps; HBM traffic 16 B per point per 20 steps)")
 
 
+C5_SEED = 0x5EED0005
+BF16_DENSE_PEAK_TFS = 2500.0
+
+
+def c5_param(A, w, delta):
+    """The polariton model above with pump amplitude A, waist w and
+    detuning delta = dc = dx."""
+    p = polariton_param()
+    p.update(A=float(A), w=float(w), dc=float(delta), dx=float(delta))
+    return p
+
+
+def c5_draw(n, rng):
+    return [c5_param(rng.uniform(0.5, 2.5), rng.uniform(10, 40), rng.uniform(0, 1.5)) for _ in range(n)]
+
+
+def c5_features(params, n_grid=256, L=L_BOX):
+    """Input maps of the surrogate: pump profile A exp(-|r - c|^2 / w^2),
+    detuning, x / L, y / L (float32, (n, 4, N, N), axis 0 of a plane = x)."""
+    x = np.arange(n_grid) * (L / n_grid)
+    X = np.empty((len(params), 4, n_grid, n_grid), dtype=np.float32)
+    r2 = (x[:, None] - L / 2) ** 2 + (x[None, :] - L / 2) ** 2
+    for i, p in enumerate(params):
+        X[i, 0] = p["A"] * np.exp(-r2 / p["w"] ** 2)
+        X[i, 1] = p["dc"]
+    X[:, 2] = (x / L)[:, None]
+    X[:, 3] = (x / L)[None, :]
+    return X
+
+
+def band_residual(u, m):
+    """||u - P u|| / ||u|| for the complex fields u (tuple of (N, N)), P the
+    retained-mode low-pass applied to Re and Im planes (kx in {0..m-1,
+    N-m..N-1}, ky in {0..m-1}, Hermitian completion): how much of a steady
+    state lies outside the surrogate's band."""
+    num = den = 0.0
+    for f in u:
+        for plane in (f.real, f.imag):
+            N = plane.shape[0]
+            F = np.fft.fft2(plane)
+            kx = np.r_[np.arange(m), np.arange(N - m, N)]
+            G = np.zeros_like(F)
+            G[kx, :m] = F[kx, :m] * np.where(np.arange(m) == 0, 1.0, 2.0)
+            num += np.linalg.norm(plane - np.fft.ifft2(G).real) ** 2
+            den += np.linalg.norm(plane) ** 2
+    return math.sqrt(num / den)
+
+
+def band_limited(u, m):
+    """The retained-mode low-pass of band_residual, applied to complex fields."""
+    out = []
+    for f in u:
+        parts = []
+        for plane in (f.real, f.imag):
+            N = plane.shape[0]
+            F = np.fft.fft2(plane)
+            kx = np.r_[np.arange(m), np.arange(N - m, N)]
+            G = np.zeros_like(F)
+            G[kx, :m] = F[kx, :m] * np.where(np.arange(m) == 0, 1.0, 2.0)
+            parts.append(np.fft.ifft2(G).real)
+        out.append(parts[0] + 1j * parts[1])
+    return tuple(out)
+
+
+def warmstart_diagnosis(it, ws, Xte, params, torch):
+    """Steps to residual <= 1e-8 from controlled starts around the converged
+    state u*: the network's prediction with and without the band limit, u*
+    band-limited, u* (1 + 1e-2), and u* plus broadband noise of relative
+    size 1e-3.  Separates the error the refinement pays for."""
+    rows = []
+    for i, p in enumerate(params):
+        it.set_param(p, ("dispersion", "pump"))
+        us, _ = it.steady_state(tol=1e-10, check_every=50, max_steps=400000)
+        nrm = math.sqrt(sum(np.linalg.norm(x) ** 2 for x in us))
+        r = np.random.default_rng(i)
+        noise = [r.standard_normal(x.shape) + 1j * r.standard_normal(x.shape) for x in us]
+        s = 1e-3 * nrm / math.sqrt(sum(np.linalg.norm(x) ** 2 for x in noise))
+        starts = {
+            "cold": None,
+            "fno_band_limited": tuple(x[0] for x in ws.predict_fields(Xte[i:i + 1], dtype=torch.complex128)),
+            "fno_raw": tuple(x[0] for x in ws.predict_fields(Xte[i:i + 1], dtype=torch.complex128,
+                                                             band_limit=False)),
+            "exact_band_limited": band_limited(us, ws.net.m),
+            "exact_x_1.01": tuple(x * 1.01 for x in us),
+            "exact_plus_1e-3_broadband": tuple(x + s * n for x, n in zip(us, noise)),
+        }
+        row = dict(A=p["A"], w=p["w"], delta=p["dc"], out_of_band=band_residual(us, ws.net.m))
+        for k, u0 in starts.items():
+            _, inf = it.steady_state(u0=u0, tol=1e-8, check_every=50, max_steps=400000)
+            row[k] = inf["steps"]
+        rows.append(row)
+        print(f"  controlled starts {row}", flush=True)
+    return rows
+
+
+def c5_problem(g, dtype, n_grid=256, L=L_BOX):
+    def make(p):
+        u0 = (np.zeros((n_grid, n_grid), dtype), np.zeros((n_grid, n_grid), dtype))
+        return g.GrossPitaevskiiProblem(u0, (L, L), dispersion=dispersion, nonlinearity=nonlinearity, pump=pump,
+                                        param=p)
+    return make
+
+
+def warmstart_run(g, torch, args):
+    """Learned warm start on the polariton model at 256x256 (L = 100): steady
+    states for n_train drawn (A ~ U[0.5, 2.5], w ~ U[10, 40], delta ~ U[0, 1.5];
+    seed 0x5EED0005) from u0 = 0 in FP32 (tol 3e-6, 8 handles in flight),
+    FNO (width 32, 16 modes, 4 layers) trained on them, then n_test unseen
+    cases run to residual <= 1e-8 in FP64 twice: cold from u0 = 0 and warm
+    from the FNO prediction.  Time = device time of the refinement (+ the
+    FNO inference for warm), both checked every 50 steps."""
+    from gpe_hip_amd import warmstart as W
+
+    rng = np.random.default_rng(C5_SEED)
+    n_tr, n_te = args.ws_train, args.ws_test
+    p_train = c5_draw(n_tr, rng)
+    p_test = c5_draw(n_te, rng)
+    N = 256
+    rec = {}
+    if args.ws_load:
+        ws = W.WarmStart.load(args.ws_load)
+        rec["train"] = dict(loaded=args.ws_load)
+    else:
+        print(f"[warmstart] generating {n_tr} FP32 steady states", flush=True)
+        t0 = time.perf_counter()
+        fields, info = W.generate_steady_states(c5_problem(g, np.complex64), p_train, dt=0.05, tol=3e-6,
+                                                check_every=200, max_steps=40000, workers=8,
+                                                progress=max(1, n_tr // 8))
+        t_gen = time.perf_counter() - t0
+        conv = np.array([i["residual"] <= 3e-6 for i in info])
+        rec["data"] = dict(n=n_tr, seconds=t_gen, ms_per_case=1e3 * t_gen / n_tr, tol=3e-6, dtype="c64",
+                           converged=int(conv.sum()), steps_mean=float(np.mean([i["steps"] for i in info])),
+                           residual_max=float(max(i["residual"] for i in info)))
+        print(f"[warmstart] data: {rec['data']}", flush=True)
+        keep = np.nonzero(np.array([i["residual"] <= 1e-4 for i in info]))[0]
+        X = torch.tensor(c5_features([p_train[i] for i in keep], N), device="cuda:0")
+        Y = torch.tensor(W.fields_to_channels(fields[keep]), device="cuda:0")
+        del fields
+        ws = W.WarmStart(N, 4, 4, width=32, modes=16, layers=4, seed=1)
+        print(f"[warmstart] training on {len(keep)} cases", flush=True)
+        t0 = time.perf_counter()
+        losses = ws.fit(X, Y, epochs=args.ws_epochs, batch=16, lr=args.ws_lr, time_budget=args.ws_time)
+        t_train = time.perf_counter() - t0
+        rec["train"] = dict(n=int(len(keep)), epochs_run=len(losses), epochs=args.ws_epochs, seconds=t_train,
+                            loss_first=losses[0], loss_last=losses[-1])
+        del X, Y
+        if args.ws_save:
+            ws.save(args.ws_save)
+    # inference roofline: one case at a time, as in the warm start
+    Xte = torch.tensor(c5_features(p_test, N), device="cuda:0")
+    ops = ws.net.ops
+    ws.predict(Xte[:1])
+    torch.cuda.synchronize()
+    f0 = ops.flops
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        ws.predict(Xte[:1])
+    e1.record()
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / reps
+    flops = (ops.flops - f0) / reps
+    rec["fno_inference"] = dict(ms=inf_ms, flops=flops, achieved_TFs=flops / (inf_ms * 1e-3) / 1e12,
+                                peak_TFs=BF16_DENSE_PEAK_TFS,
+                                frac=flops / (inf_ms * 1e-3) / 1e12 / BF16_DENSE_PEAK_TFS)
+    print(f"[warmstart] inference {rec['fno_inference']}", flush=True)
+    # refinement: cold vs warm, FP64, one handle
+    it = g.init(c5_problem(g, np.complex128)(p_test[0]), g.StrangSplitting(), (0.0, 0.05), dt=0.05, nsaves=1)
+    cold_ms, warm_ms, cold_steps, warm_steps, err, same, outband = [], [], [], [], [], [], []
+    t0 = time.perf_counter()
+    for i, p in enumerate(p_test):
+        it.set_param(p, ("dispersion", "pump"))
+        uc, ic = it.steady_state(tol=1e-8, check_every=50, max_steps=400000)
+        ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ea.record()
+        pred = ws.predict_fields(Xte[i:i + 1], dtype=torch.complex128)
+        eb.record()
+        torch.cuda.synchronize()
+        t_inf = ea.elapsed_time(eb)
+        uw, iw = it.steady_state(u0=tuple(x[0] for x in pred), tol=1e-8, check_every=50, max_steps=400000)
+        pn = [x[0].cpu().numpy() for x in pred]
+        nrm = math.sqrt(sum(np.linalg.norm(uc[c]) ** 2 for c in range(2)))
+        err.append(math.sqrt(sum(np.linalg.norm(pn[c] - uc[c]) ** 2 for c in range(2))) / nrm)
+        same.append(math.sqrt(sum(np.linalg.norm(uw[c] - uc[c]) ** 2 for c in range(2))) / nrm)
+        outband.append(band_residual(uc, ws.net.m))
+        cold_ms.append(ic["ms"])
+        warm_ms.append(iw["ms"] + t_inf)
+        cold_steps.append(ic["steps"])
+        warm_steps.append(iw["steps"])
+        if (i + 1) % max(1, n_te // 8) == 0:
+            print(f"  refined {i + 1}/{n_te}  cold {np.mean(cold_ms):.1f} ms  warm {np.mean(warm_ms):.1f} ms  "
+                  f"fno err median {np.median(err):.2e}  ({time.perf_counter() - t0:.0f} s)", flush=True)
+    diag = warmstart_diagnosis(it, ws, Xte, p_test[:args.ws_diag], torch) if args.ws_diag else None
+    it.close()
+    cold_ms, warm_ms = np.array(cold_ms), np.array(warm_ms)
+    err, same = np.array(err), np.array(same)
+    out = {
+        "metric": "time to steady state (residual <= 1e-8), cold / FNO-warm",
+        "value": float(cold_ms.sum() / warm_ms.sum()),
+        "unit": "x (ratio of summed device times)",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "refine c128; FNO bf16 x bf16 -> f32",
+        "data": "synthetic: steady states generated by this solver",
+        "config": {"workload": "polariton model 256x256, L=100, A~U[0.5,2.5], w~U[10,40], delta~U[0,1.5] "
+                               "(seed 0x5EED0005); FNO width 32, 16 modes, 4 layers",
+                   "n_train": n_tr, "n_test": n_te, "check_every": 50},
+        "cold_ms_mean": float(cold_ms.mean()), "warm_ms_mean": float(warm_ms.mean()),
+        "cold_steps_mean": float(np.mean(cold_steps)), "warm_steps_mean": float(np.mean(warm_steps)),
+        "per_case_speedup_median": float(np.median(cold_ms / warm_ms)),
+        "fno_rel_err": {"median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
+                        "max": float(err.max())},
+        "warm_vs_cold_final_rel_diff_max": float(same.max()),
+        "steady_state_out_of_band_max": float(np.max(outband)),
+        "cases_other_branch": int((same > 1e-4).sum()),
+        "controlled_starts": diag,
+        "roofline": {"bound": "mfma", "kernel": "gemm_kernel<bf16> (FNO inference, one case)",
+                     "achieved": rec["fno_inference"]["achieved_TFs"], "peak": BF16_DENSE_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": rec["fno_inference"]["frac"], "traffic": None},
+        **rec,
+    }
+    print(json.dumps(out))
+
+
 def cpu_baseline(seconds=12.0):
     """FP64 numpy oracle (one host core) on a bounded sample of the same
     workload: as many steps as fit in about `seconds`, from the state after a
@@ -257,6 +482,16 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the uneven-size sweep line")
     ap.add_argument("--dtype", default="complex64", choices=["complex64", "complex128"],
                     help="field precision of the headline run")
+    ap.add_argument("--warmstart", action="store_true",
+                    help="run only the learned warm-start experiment (data, training, cold vs warm refinement)")
+    ap.add_argument("--ws-train", type=int, default=4096)
+    ap.add_argument("--ws-test", type=int, default=512)
+    ap.add_argument("--ws-epochs", type=int, default=40)
+    ap.add_argument("--ws-lr", type=float, default=1e-3)
+    ap.add_argument("--ws-time", type=float, default=420.0, help="training time budget in seconds")
+    ap.add_argument("--ws-save", default=None, help="write the trained surrogate (.npz)")
+    ap.add_argument("--ws-load", default=None, help="skip data and training, load a surrogate (.npz)")
+    ap.add_argument("--ws-diag", type=int, default=0, help="controlled-start comparison on this many test cases")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (PMC calibration)")
     args = ap.parse_args()
@@ -278,6 +513,9 @@ def main():
     import __graft_entry__ as ge
 
     g = ge.load_package()
+    if args.warmstart:
+        warmstart_run(g, torch, args)
+        return
     runs = {}
     for dtype in ([args.dtype] + [d for d in ("complex64", "complex128") if d != args.dtype]):
         runs[dtype] = gpu_run(g, torch, dtype, args.steps, args.warmup)

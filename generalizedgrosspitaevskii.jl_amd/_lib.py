@@ -88,6 +88,7 @@ SIGNATURES = {
                                          ctypes.POINTER(I64), D, P, I64, ctypes.POINTER(I64),
                                          ctypes.POINTER(ctypes.c_float)]),
     "gpe_fno_gemm": (ctypes.c_int, [P, P, P, I64, I64, I64, P, I64, ctypes.c_float, ctypes.c_float, P]),
+    "gpe_fno_gemm_f32": (ctypes.c_int, [P, P, P, I64, I64, I64, P, I64, ctypes.c_float, ctypes.c_float, P]),
     "gpe_fno_bias_act": (ctypes.c_int, [P, P, P, P, I64, I64, I64, I32, I32, P]),
     "gpe_fno_act_backward": (ctypes.c_int, [P, P, P, I64, P]),
     "gpe_fno_axpby": (ctypes.c_int, [P, P, ctypes.c_float, ctypes.c_float, I64, P]),

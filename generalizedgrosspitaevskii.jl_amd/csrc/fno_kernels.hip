@@ -8,12 +8,17 @@
 //                    accumulation.  Every dense product of the network and
 //                    of its backward pass (truncated DFTs, spectral channel
 //                    mixing, 1x1 convolutions, lifting, projection) is one.
+//   gpe_fno_gemm_f32 the same product with FP32 operands
+//                    (v_mfma_f32_16x16x4_f32): the band-limiting transforms
+//                    of a prediction, where BF16-rounded basis functions
+//                    would leave broadband error.
 //   gpe_fno_bias_act bias add + GELU (tanh form) and its backward.
 //   gpe_fno_adam     Adam update of FP32 parameters.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "../../include/gpe_hip.h"
 
@@ -45,16 +50,32 @@ struct GemmArgs {
 };
 
 // 256 threads = 4 wavefronts; the workgroup computes a 64 x 64 tile of C,
-// each wavefront a 32 x 32 quarter as 2 x 2 MFMA 16x16x16 blocks.  A and B
-// tiles of depth 16 are staged through LDS as BF16.
+// each wavefront a 32 x 32 quarter as 2 x 2 MFMA 16x16 blocks.  A and B
+// tiles of depth 16 are staged through LDS (BF16 or FP32 per kBF16).
 //
 // MFMA 16x16x16 BF16 operand layout (lane l of the wavefront):
 //   A fragment: row l % 16, k = 4 (l / 16) .. 4 (l / 16) + 3
 //   B fragment: k = 4 (l / 16) .. +3, column l % 16
-//   D: rows 4 (l / 16) .. +3, column l % 16
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-  __shared__ short As[TM][TK + 4];
-  __shared__ short Bs[TK][TN + 4];
+// MFMA 16x16x4 FP32: A row l % 16, k = l / 16; B k = l / 16, column l % 16.
+// Both: D rows 4 (l / 16) .. +3, column l % 16.
+template <bool kBF16>
+struct GemmLds {
+  using T = typename std::conditional<kBF16, short, float>::type;
+  T As[TM][TK + 4];
+  T Bs[TK][TN + 4];
+};
+
+template <bool kBF16>
+__device__ __forceinline__ typename GemmLds<kBF16>::T lds_value(float v) {
+  if constexpr (kBF16)
+    return to_bf16(v);
+  else
+    return v;
+}
+
+template <bool kBF16>
+__global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
+  __shared__ GemmLds<kBF16> L;
   const int64_t bz = blockIdx.z;
   const float* A = g.A + bz * g.sa;
   const float* B = g.B + bz * g.sb;
@@ -73,7 +94,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       const int64_t gi = m0 + r, gk = k0 + kk;
       float v = 0.f;
       if (gi < g.m && gk < g.k) v = A[gi * g.ars + gk * g.acs];
-      As[r][kk] = to_bf16(v);
+      L.As[r][kk] = lds_value<kBF16>(v);
     }
     // stage B: 16 x 64 values
     for (int e = tid; e < TK * TN; e += 256) {
@@ -81,25 +102,38 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       const int64_t gk = k0 + kk, gj = n0 + c;
       float v = 0.f;
       if (gk < g.k && gj < g.n) v = B[gk * g.brs + gj * g.bcs];
-      Bs[kk][c] = to_bf16(v);
+      L.Bs[kk][c] = lds_value<kBF16>(v);
     }
     __syncthreads();
-    const int kb = 4 * (lane / 16);
-    for (int i = 0; i < 2; ++i) {
-      bf16x4 a;
-      const int r = wm + 16 * i + lane % 16;
-      a[0] = As[r][kb];
-      a[1] = As[r][kb + 1];
-      a[2] = As[r][kb + 2];
-      a[3] = As[r][kb + 3];
-      for (int j = 0; j < 2; ++j) {
-        bf16x4 b;
-        const int c = wn + 16 * j + lane % 16;
-        b[0] = Bs[kb][c];
-        b[1] = Bs[kb + 1][c];
-        b[2] = Bs[kb + 2][c];
-        b[3] = Bs[kb + 3][c];
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+    if constexpr (kBF16) {
+      const int kb = 4 * (lane / 16);
+      for (int i = 0; i < 2; ++i) {
+        bf16x4 a;
+        const int r = wm + 16 * i + lane % 16;
+        a[0] = L.As[r][kb];
+        a[1] = L.As[r][kb + 1];
+        a[2] = L.As[r][kb + 2];
+        a[3] = L.As[r][kb + 3];
+        for (int j = 0; j < 2; ++j) {
+          bf16x4 b;
+          const int c = wn + 16 * j + lane % 16;
+          b[0] = L.Bs[kb][c];
+          b[1] = L.Bs[kb + 1][c];
+          b[2] = L.Bs[kb + 2][c];
+          b[3] = L.Bs[kb + 3][c];
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+        }
+      }
+    } else {
+      for (int ks = 0; ks < TK; ks += 4) {
+        const int kk = ks + lane / 16;
+        for (int i = 0; i < 2; ++i) {
+          const float a = L.As[wm + 16 * i + lane % 16][kk];
+          for (int j = 0; j < 2; ++j) {
+            const float b = L.Bs[kk][wn + 16 * j + lane % 16];
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
+          }
+        }
       }
     }
     __syncthreads();
@@ -190,14 +224,26 @@ extern "C" int gpe_fno_axpby(const float* x, float* y, float a, float b, int64_t
   return hipGetLastError() == hipSuccess ? GPE_OK : GPE_ERR_DEVICE;
 }
 
-extern "C" int gpe_fno_gemm(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k,
-                            const int64_t* strides, int64_t batch, float alpha, float beta, void* stream) {
+template <bool kBF16>
+static int launch_gemm(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k,
+                       const int64_t* strides, int64_t batch, float alpha, float beta, void* stream) {
   if (!A || !B || !C || !strides || m < 1 || n < 1 || k < 1 || batch < 1 || batch > 65535) return GPE_ERR_INVALID;
+  if ((m + TM - 1) / TM > 65535) return GPE_ERR_INVALID;  // grid.y limit
   GemmArgs g{A, B, C, m, n, k, strides[0], strides[1], strides[2], strides[3], strides[4], strides[5],
              strides[6], strides[7], strides[8], alpha, beta};
   const dim3 grid((unsigned)((n + TN - 1) / TN), (unsigned)((m + TM - 1) / TM), (unsigned)batch);
-  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, g);
+  hipLaunchKernelGGL(gemm_kernel<kBF16>, grid, dim3(256), 0, (hipStream_t)stream, g);
   return hipGetLastError() == hipSuccess ? GPE_OK : GPE_ERR_DEVICE;
+}
+
+extern "C" int gpe_fno_gemm(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k,
+                            const int64_t* strides, int64_t batch, float alpha, float beta, void* stream) {
+  return launch_gemm<true>(A, B, C, m, n, k, strides, batch, alpha, beta, stream);
+}
+
+extern "C" int gpe_fno_gemm_f32(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k,
+                                const int64_t* strides, int64_t batch, float alpha, float beta, void* stream) {
+  return launch_gemm<false>(A, B, C, m, n, k, strides, batch, alpha, beta, stream);
 }
 
 extern "C" int gpe_fno_bias_act(const float* x, const float* bias, float* pre, float* y, int64_t rows, int64_t cols,

@@ -41,6 +41,16 @@ static int fail(int code, const char* fmt, ...) {
   } while (0)
 
 // ------------------------------------------------------------------ handle
+// Scratch of gpe_steady_state: two field snapshots and the reduction buffers,
+// allocated on the first call and kept by the handle.
+struct SteadyBufs {
+  void* a[kMaxM] = {nullptr, nullptr};
+  void* b[kMaxM] = {nullptr, nullptr};
+  double* partial = nullptr;
+  double* result = nullptr;
+  double* host = nullptr;  // pinned pair (diff^2, norm^2)
+};
+
 struct gpe_handle {
   gpe_problem_desc d;
   int M = 1, ndim = 1;
@@ -57,6 +67,7 @@ struct gpe_handle {
   int64_t ed_e0 = 0, ed_e1 = 0;   // compact table extents
   double* dD = nullptr;
   void* ed = nullptr;
+  size_t dD_cap = 0, ed_cap = 0;  // bytes held (tables are reused while they fit)
   double ed_dt = NAN;
   // potential
   int pot_form = GPE_FORM_NONE;
@@ -71,8 +82,11 @@ struct gpe_handle {
   double c[kMaxM][2] = {};
   // pump
   int pump_kind = GPE_PUMP_NONE;
-  void* pump_a[kMaxM] = {nullptr, nullptr};
+  void* pump_a[kMaxM] = {nullptr, nullptr};  // active profile per component (nullptr: F_c = 0)
   void* pump_b[kMaxM] = {nullptr, nullptr};
+  void* pump_buf_a[kMaxM] = {nullptr, nullptr};  // owned storage behind pump_a / pump_b
+  void* pump_buf_b[kMaxM] = {nullptr, nullptr};
+  size_t pump_cap_a[kMaxM] = {0, 0}, pump_cap_b[kMaxM] = {0, 0};
   std::vector<double> env;  // complex samples
   double* d_env = nullptr;
   int64_t d_env_cap = 0;
@@ -93,6 +107,7 @@ struct gpe_handle {
   float last_ms = 0;
   double event_pair_ms = 0;  // latency of an empty HIP event pair (gpe_profile_kernels)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  SteadyBufs steady;
 };
 
 static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -130,6 +145,37 @@ static int upload(void** dst, const void* src, size_t bytes) {
 static void dfree(void*& p) {
   if (p) (void)hipFree(p);
   p = nullptr;
+}
+
+// Keep *p when it already holds `bytes`; reallocate otherwise.  Re-setting a
+// term with a table of the same size then allocates and frees nothing.
+static int ensure_buf(void** p, size_t* cap, size_t bytes) {
+  if (*p && *cap >= bytes) return GPE_OK;
+  dfree(*p);
+  *cap = 0;
+  HIPCHK(hipMalloc(p, bytes));
+  *cap = bytes;
+  return GPE_OK;
+}
+
+static int upload_into(void** p, size_t* cap, const void* src, size_t bytes) {
+  int rc = ensure_buf(p, cap, bytes);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(*p, src, bytes, hipMemcpyHostToDevice));
+  return GPE_OK;
+}
+
+static void free_steady(SteadyBufs& sb) {
+  for (int c = 0; c < kMaxM; ++c) {
+    dfree(sb.a[c]);
+    dfree(sb.b[c]);
+  }
+  void* p = sb.partial;
+  dfree(p);
+  p = sb.result;
+  dfree(p);
+  if (sb.host) (void)hipHostFree(sb.host);
+  sb = SteadyBufs();
 }
 
 // Twiddle table w[m] = exp(-2 pi i m / n), m = 0..n-1, in FP64: the angle is
@@ -234,8 +280,8 @@ extern "C" int gpe_destroy(gpe_handle* h) {
   for (int c = 0; c < kMaxM; ++c) {
     dfree(h->u[c]);
     dfree(h->ev[c]);
-    dfree(h->pump_a[c]);
-    dfree(h->pump_b[c]);
+    dfree(h->pump_buf_a[c]);
+    dfree(h->pump_buf_b[c]);
     dfree(h->stage[c]);
   }
   dfree(h->tw);
@@ -248,6 +294,7 @@ extern "C" int gpe_destroy(gpe_handle* h) {
   dfree(h->ed);
   p = h->d_env;
   dfree(p);
+  free_steady(h->steady);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -320,14 +367,16 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
         for (int i = 0; i < 2 * per_out; ++i) dst[i] = src[i];
       }
     }
+  // launches already queued on the stream may still read the current tables
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->disp_form = GPE_FORM_NONE;
+  h->ed_dt = NAN;
   void* p = h->dD;
-  dfree(p);
-  h->dD = nullptr;
-  dfree(h->ed);
-  int rc = upload(&p, compact.data(), compact.size() * sizeof(double));
+  int rc = upload_into(&p, &h->dD_cap, compact.data(), compact.size() * sizeof(double));
   h->dD = (double*)p;
   if (rc) return rc;
-  HIPCHK(hipMalloc(&h->ed, (size_t)e0 * e1 * per_out * h->csz));
+  rc = ensure_buf(&h->ed, &h->ed_cap, (size_t)e0 * e1 * per_out * h->csz);
+  if (rc) return rc;
   h->disp_form = dform;
   h->ed_fold = fold;
   h->ed_e0 = e0;
@@ -339,6 +388,7 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
 extern "C" int gpe_set_potential(gpe_handle* h, int32_t form, const double* V) {
   if (!h) return fail(GPE_ERR_INVALID, "gpe_set_potential: null handle");
   HIPCHK(hipSetDevice(h->d.device));
+  HIPCHK(hipStreamSynchronize(h->stream));  // queued launches may still read the tables
   void* p = h->dV;
   dfree(p);
   h->dV = nullptr;
@@ -389,9 +439,10 @@ extern "C" int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double
 extern "C" int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, const double* envelope, int64_t n_env) {
   if (!h) return fail(GPE_ERR_INVALID, "gpe_set_pump: null handle");
   HIPCHK(hipSetDevice(h->d.device));
+  HIPCHK(hipStreamSynchronize(h->stream));  // queued launches may still read the profiles
   for (int c = 0; c < kMaxM; ++c) {
-    dfree(h->pump_a[c]);
-    dfree(h->pump_b[c]);
+    h->pump_a[c] = nullptr;
+    h->pump_b[c] = nullptr;
   }
   h->env.clear();
   h->pump_kind = GPE_PUMP_NONE;
@@ -406,20 +457,22 @@ extern "C" int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, 
     for (int64_t i = 0; i < 2 * per && !nonzero; ++i) nonzero = pc[i] != 0.0;
     if (!nonzero) continue;  // component not pumped: F_c = 0 exactly
     std::vector<double> tmp;
-    auto put = [&](void** dst, const double* src, int64_t n) -> int {
-      if (h->f64) return upload(dst, src, (size_t)n * 16);
+    auto put = [&](void** dst, size_t* cap, const double* src, int64_t n) -> int {
+      if (h->f64) return upload_into(dst, cap, src, (size_t)n * 16);
       std::vector<float> f(2 * n);
       for (int64_t i = 0; i < 2 * n; ++i) f[i] = (float)src[i];
-      return upload(dst, f.data(), (size_t)n * 8);
+      return upload_into(dst, cap, f.data(), (size_t)n * 8);
     };
     int rc;
     if (kind == GPE_PUMP_TABLE) {
-      rc = put(&h->pump_a[c], pc, h->nsp);
+      rc = put(&h->pump_buf_a[c], &h->pump_cap_a[c], pc, h->nsp);
     } else {
-      rc = put(&h->pump_a[c], pc, h->n0);
-      if (!rc && h->ndim == 2) rc = put(&h->pump_b[c], pc + 2 * h->n0, h->n1);
+      rc = put(&h->pump_buf_a[c], &h->pump_cap_a[c], pc, h->n0);
+      if (!rc && h->ndim == 2) rc = put(&h->pump_buf_b[c], &h->pump_cap_b[c], pc + 2 * h->n0, h->n1);
     }
     if (rc) return rc;
+    h->pump_a[c] = h->pump_buf_a[c];
+    h->pump_b[c] = (kind == GPE_PUMP_SEPARABLE && h->ndim == 2) ? h->pump_buf_b[c] : nullptr;
   }
   h->env.assign(envelope, envelope + 2 * n_env);
   h->pump_kind = kind;
@@ -1252,26 +1305,6 @@ extern "C" int gpe_sweep_last_ms(gpe_sweep* s, float* ms) {
 // ------------------------------------------------------------ steady state
 // Step from u0 until r = ||u_{n+1} - u_n|| / ||u_{n+1}|| <= tol, checking
 // every `check_every` steps on two consecutive real-space fields.
-struct SteadyBufs {
-  void* a[kMaxM] = {nullptr, nullptr};
-  void* b[kMaxM] = {nullptr, nullptr};
-  double* partial = nullptr;
-  double* result = nullptr;
-  double* host = nullptr;  // pinned pair (diff^2, norm^2)
-};
-
-static void free_steady(SteadyBufs& sb) {
-  for (int c = 0; c < kMaxM; ++c) {
-    dfree(sb.a[c]);
-    dfree(sb.b[c]);
-  }
-  void* p = sb.partial;
-  dfree(p);
-  p = sb.result;
-  dfree(p);
-  if (sb.host) (void)hipHostFree(sb.host);
-  sb.host = nullptr;
-}
 
 extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt, double tol, int64_t check_every,
                                 int64_t max_steps, int64_t* steps, double* residual, double* history,
@@ -1285,19 +1318,21 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
   int rc = prepare_tables(h, dt);
   if (rc) return rc;
   const size_t fb = field_bytes(h);
-  SteadyBufs sb;
+  SteadyBufs& sb = h->steady;
   const int nblk = 512;
-  auto cleanup = [&](int code) {
-    free_steady(sb);
-    return code;
-  };
-  for (int c = 0; c < h->M; ++c)
-    if (hipMalloc(&sb.a[c], fb) != hipSuccess || hipMalloc(&sb.b[c], fb) != hipSuccess)
-      return cleanup(fail(GPE_ERR_NOMEM, "gpe_steady_state: field buffers"));
-  if (hipMalloc((void**)&sb.partial, 4 * nblk * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&sb.result, 2 * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void**)&sb.host, 2 * sizeof(double)) != hipSuccess)
-    return cleanup(fail(GPE_ERR_NOMEM, "gpe_steady_state: reduction buffers"));
+  auto cleanup = [&](int code) { return code; };
+  if (!sb.host) {  // first call on this handle: allocate the scratch it keeps
+    bool ok = true;
+    for (int c = 0; c < h->M && ok; ++c)
+      ok = hipMalloc(&sb.a[c], fb) == hipSuccess && hipMalloc(&sb.b[c], fb) == hipSuccess;
+    ok = ok && hipMalloc((void**)&sb.partial, 4 * nblk * sizeof(double)) == hipSuccess &&
+         hipMalloc((void**)&sb.result, 2 * sizeof(double)) == hipSuccess &&
+         hipHostMalloc((void**)&sb.host, 2 * sizeof(double)) == hipSuccess;
+    if (!ok) {
+      free_steady(sb);
+      return fail(GPE_ERR_NOMEM, "gpe_steady_state: scratch buffers");
+    }
+  }
   for (int c = 0; c < h->M; ++c)
     if (hipMemcpyAsync(h->u[c], u0[c], fb, hipMemcpyDefault, h->stream) != hipSuccess)
       return cleanup(fail(GPE_ERR_DEVICE, "gpe_steady_state: upload"));

@@ -41,12 +41,16 @@ class _Ops:
     def __init__(self, stream=None):
         self.lib = L.load()
         self.stream = stream
+        self.flops = 0  # multiply-adds x 2 issued through mm (roofline accounting)
 
-    def mm(self, A, B, C, m, n, k, a, b, c, batch=1, alpha=1.0, beta=0.0, offs=(0, 0, 0)):
-        """C = alpha A B + beta C; a/b/c = (row stride, col stride, batch stride)."""
+    def mm(self, A, B, C, m, n, k, a, b, c, batch=1, alpha=1.0, beta=0.0, offs=(0, 0, 0), fp32=False):
+        """C = alpha A B + beta C; a/b/c = (row stride, col stride, batch stride);
+        BF16 operands unless fp32."""
         st = (ctypes.c_int64 * 9)(a[0], a[1], b[0], b[1], c[0], c[1], a[2], b[2], c[2])
-        L.check(self.lib.gpe_fno_gemm(_ptr(A, offs[0]), _ptr(B, offs[1]), _ptr(C, offs[2]), m, n, k, st, batch,
-                                      float(alpha), float(beta), self.stream))
+        self.flops += 2 * m * n * k * batch
+        fn = self.lib.gpe_fno_gemm_f32 if fp32 else self.lib.gpe_fno_gemm
+        L.check(fn(_ptr(A, offs[0]), _ptr(B, offs[1]), _ptr(C, offs[2]), m, n, k, st, batch, float(alpha),
+                   float(beta), self.stream))
 
     def bias_act(self, x, bias, pre, y, rows, cols, C, act):
         L.check(self.lib.gpe_fno_bias_act(_ptr(x), None if bias is None else _ptr(bias),
@@ -127,22 +131,57 @@ class FNO:
             self.ones = self.torch.ones(n, dtype=self.torch.float32, device=self.dev)
 
     # ------------------------------------------------------- spectral op
-    def _spectral(self, h, l, out, B, save):
-        """out (B*W, N, N) += K_l(h)."""
-        o, N, m, W, Q = self.ops, self.N, self.m, self.W, self.Q
-        P = B * W
+    def _dft_truncated(self, h, P, fp32=False):
+        """(Yr, Yi), each (P, Q): the DFT of P real N x N planes at the
+        retained modes (kx in {0..m-1, N-m..N-1}, ky in {0..m-1})."""
+        o, N, m, Q = self.ops, self.N, self.m, self.Q
         c = self.c
         Z = self._buf(P, N, 2 * m)
-        o.mm(h, c["Fy"], Z, P * N, 2 * m, N, (N, 1, 0), (2 * m, 1, 0), (2 * m, 1, 0))
+        o.mm(h, c["Fy"], Z, P * N, 2 * m, N, (N, 1, 0), (2 * m, 1, 0), (2 * m, 1, 0), fp32=fp32)
         Yr, Yi = self._buf(P, Q), self._buf(P, Q)
         # Yr = Fxr Zr - Fxi Zi ; Yi = Fxr Zi + Fxi Zr   (per plane)
         zr, zi = (2 * m, 1, N * 2 * m), (2 * m, 1, N * 2 * m)
         fx = (N, 1, 0)
         yo = (m, 1, Q)
-        o.mm(c["Fxr"], Z, Yr, 2 * m, m, N, fx, zr, yo, batch=P)
-        o.mm(c["Fxi"], Z, Yr, 2 * m, m, N, fx, zi, yo, batch=P, alpha=-1.0, beta=1.0, offs=(0, m, 0))
-        o.mm(c["Fxr"], Z, Yi, 2 * m, m, N, fx, zi, yo, batch=P, offs=(0, m, 0))
-        o.mm(c["Fxi"], Z, Yi, 2 * m, m, N, fx, zr, yo, batch=P, beta=1.0)
+        o.mm(c["Fxr"], Z, Yr, 2 * m, m, N, fx, zr, yo, batch=P, fp32=fp32)
+        o.mm(c["Fxi"], Z, Yr, 2 * m, m, N, fx, zi, yo, batch=P, alpha=-1.0, beta=1.0, offs=(0, m, 0), fp32=fp32)
+        o.mm(c["Fxr"], Z, Yi, 2 * m, m, N, fx, zi, yo, batch=P, offs=(0, m, 0), fp32=fp32)
+        o.mm(c["Fxi"], Z, Yi, 2 * m, m, N, fx, zr, yo, batch=P, beta=1.0, fp32=fp32)
+        return Yr, Yi
+
+    def _idft_truncated(self, Or, Oi, P, out, beta, fp32=False):
+        """out (P, N, N) = beta out + (1/N^2) sum_modes c_ky Re[O e^{+i(kx x + ky y)}]."""
+        o, N, m, Q = self.ops, self.N, self.m, self.Q
+        c = self.c
+        Vr, Vi = self._buf(P, N, m), self._buf(P, N, m)
+        gx = (2 * m, 1, 0)
+        ob = (m, 1, Q)
+        vc = (m, 1, N * m)
+        o.mm(c["Gxr"], Or, Vr, N, m, 2 * m, gx, ob, vc, batch=P, fp32=fp32)
+        o.mm(c["Gxi"], Oi, Vr, N, m, 2 * m, gx, ob, vc, batch=P, alpha=-1.0, beta=1.0, fp32=fp32)
+        o.mm(c["Gxr"], Oi, Vi, N, m, 2 * m, gx, ob, vc, batch=P, fp32=fp32)
+        o.mm(c["Gxi"], Or, Vi, N, m, 2 * m, gx, ob, vc, batch=P, beta=1.0, fp32=fp32)
+        o.mm(Vr, c["Gc"], out, P * N, N, m, (m, 1, 0), (N, 1, 0), (N, 1, 0), beta=beta, fp32=fp32)
+        o.mm(Vi, c["Gs"], out, P * N, N, m, (m, 1, 0), (N, 1, 0), (N, 1, 0), beta=1.0, fp32=fp32)
+
+    def band_limit(self, y):
+        """Low-pass of real planes y (..., N, N): the inverse transform of
+        their retained modes (the spectral layers' band), as a new tensor.
+        FP32 products: rounding the basis functions to BF16 would put error
+        back at every wavenumber."""
+        N = self.N
+        P = y.numel() // (N * N)
+        Yr, Yi = self._dft_truncated(y, P, fp32=True)
+        out = self._buf(*y.shape)
+        self._idft_truncated(Yr, Yi, P, out, beta=0.0, fp32=True)
+        return out
+
+    def _spectral(self, h, l, out, B, save):
+        """out (B*W, N, N) += K_l(h)."""
+        o, N, m, W, Q = self.ops, self.N, self.m, self.W, self.Q
+        P = B * W
+        c = self.c
+        Yr, Yi = self._dft_truncated(h, P)
         # per-mode mixing: O = Y R (complex), Or = Yr Wr - Yi Wi, Oi = Yr Wi + Yi Wr
         Or, Oi = self._buf(P, Q), self._buf(P, Q)
         ya = (W * Q, Q, 1)
@@ -154,18 +193,8 @@ class FNO:
         o.mm(Yi, Wi, Or, B, W, W, ya, wb, oc, batch=Q, alpha=-1.0, beta=1.0, offs=(0, woff, 0))
         o.mm(Yr, Wi, Oi, B, W, W, ya, wb, oc, batch=Q, offs=(0, woff, 0))
         o.mm(Yi, Wr, Oi, B, W, W, ya, wb, oc, batch=Q, beta=1.0, offs=(0, woff, 0))
-        # inverse along x: Vr = Gxr Or - Gxi Oi ; Vi = Gxr Oi + Gxi Or
-        Vr, Vi = self._buf(P, N, m), self._buf(P, N, m)
-        gx = (2 * m, 1, 0)
-        ob = (m, 1, Q)
-        vc = (m, 1, N * m)
-        o.mm(c["Gxr"], Or, Vr, N, m, 2 * m, gx, ob, vc, batch=P)
-        o.mm(c["Gxi"], Oi, Vr, N, m, 2 * m, gx, ob, vc, batch=P, alpha=-1.0, beta=1.0)
-        o.mm(c["Gxr"], Oi, Vi, N, m, 2 * m, gx, ob, vc, batch=P)
-        o.mm(c["Gxi"], Or, Vi, N, m, 2 * m, gx, ob, vc, batch=P, beta=1.0)
-        # inverse along y to a real field, accumulated into out
-        o.mm(Vr, c["Gc"], out, P * N, N, m, (m, 1, 0), (N, 1, 0), (N, 1, 0), beta=1.0)
-        o.mm(Vi, c["Gs"], out, P * N, N, m, (m, 1, 0), (N, 1, 0), (N, 1, 0), beta=1.0)
+        # inverse transform (x then y) to a real field, accumulated into out
+        self._idft_truncated(Or, Oi, P, out, beta=1.0)
         if save:
             return dict(Yr=Yr, Yi=Yi)
         return None

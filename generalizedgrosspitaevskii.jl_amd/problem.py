@@ -478,41 +478,63 @@ class StrangSplittingIterator:
             self.h = None
             raise
 
-    def _set_terms(self):
+    TERMS = ("dispersion", "potential", "nonlinearity", "pump", "noise")
+
+    def set_param(self, param, terms=TERMS):
+        """Replace prob.param and re-evaluate the listed terms (closures of
+        param) on this handle.  Tables of an unchanged size keep their device
+        memory, so a parameter scan over one handle allocates nothing per
+        point.  No reference counterpart (the reference builds a new problem
+        and iterator per parameter set)."""
+        unknown = set(terms) - set(self.TERMS)
+        if unknown:
+            raise ValueError(f"set_param: unknown terms {sorted(unknown)}")
+        old = self.prob
+        new = GrossPitaevskiiProblem(old.u0, old.lengths, dispersion=old.dispersion, potential=old.potential,
+                                     nonlinearity=old.nonlinearity, pump=old.pump,
+                                     position_noise_func=old.position_noise_func,
+                                     momentum_noise_func=old.momentum_noise_func,
+                                     noise_prototype=old.noise_prototype, param=param)
+        self.prob = new
+        self._set_terms(terms)
+
+    def _set_terms(self, which=TERMS):
         prob, lib, h = self.prob, self._lib, self.h
         M, spatial = prob.M, self.spatial
         kax = _bcast_axes(reciprocal_grid(prob))
         rax = _bcast_axes(direct_grid(prob))
-        self._keep = []
-        if prob.dispersion is not None:
+        self._keep = []  # the library copies every table during the set call
+        if "dispersion" not in which:
+            pass
+        elif prob.dispersion is not None:
             kind, arr = _form(prob.dispersion(kax, prob.param), spatial)
             form, tab = _table_desc(kind, arr, M, spatial, "dispersion")
             self._keep.append(tab)
             L.check(lib.gpe_set_dispersion(h, form, tab.ctypes.data))
         else:
             L.check(lib.gpe_set_dispersion(h, L.GPE_FORM_NONE, None))
-        if prob.potential is not None:
+        if "potential" in which and prob.potential is not None:
             kind, arr = _form(prob.potential(rax, prob.param), spatial)
             if kind == "M":
                 raise NotImplementedError("potential: SMatrix V(r) is not supported by libgpe_hip")
             form, tab = _table_desc(kind, arr, M, spatial, "potential")
             self._keep.append(tab)
             L.check(lib.gpe_set_potential(h, form, tab.ctypes.data))
-        if prob.nonlinearity is not None:
+        if "nonlinearity" in which and prob.nonlinearity is not None:
             g, c = fit_nonlinearity(prob.nonlinearity, M, prob.param)
             # column-major g[i + M j]
             gbuf = np.ascontiguousarray(g.T.ravel()).view(np.float64)
             cbuf = np.ascontiguousarray(c).view(np.float64)
             self._keep += [gbuf, cbuf]
             L.check(lib.gpe_set_nonlinearity(h, gbuf.ctypes.data, cbuf.ctypes.data))
-        if prob.pump is not None:
+        if "pump" in which and prob.pump is not None:
             times = pump_times(self.tspan, self.dt_in, self.nsaves)
             kind, prof, env = pump_descriptor(prob, rax, spatial, times)
             if kind != L.GPE_PUMP_NONE:
                 self._keep += [prof, env]
                 L.check(lib.gpe_set_pump(h, kind, prof.ctypes.data, None if env is None else env.ctypes.data,
                                          0 if env is None else len(env) // 2))
-        if prob.noise_prototype is not None:
+        if "noise" in which and prob.noise_prototype is not None:
             proto = prob.noise_prototype
             first = proto[0] if isinstance(proto, (tuple, list)) else proto
             cplx = np.iscomplexobj(np.empty(0, dtype=str(first.dtype).replace("torch.", "")))
@@ -549,6 +571,7 @@ class StrangSplittingIterator:
                 result = tuple(device_empty(rshape, str(x.dtype).replace("torch.", ""), x.device.index) for x in u0)
             uptr = [x.data_ptr() for x in u0]
             rptr = None if result is None else [x.data_ptr() for x in result]
+            _order_after_torch(u0[0])
         else:
             u0 = [np.asfortranarray(x) for x in prob.u0]
             if result is None and save_frames:
@@ -599,9 +622,13 @@ class StrangSplittingIterator:
         Returns (u, info) with info = dict(steps, residual, history, ms)."""
         lib = self._lib
         src = self.prob.u0 if u0 is None else tuple(u0)
-        if self.on_device:
-            u = [x if _contig_colmajor(x) else _to_colmajor(x) for x in src]
+        if _is_torch(src[0]) and src[0].is_cuda:  # device start field (either kind of iterator)
+            want = self.prob.u0[0].dtype
+            want = str(want).replace("torch.", "")
+            u = [x if (_contig_colmajor(x) and str(x.dtype).replace("torch.", "") == want)
+                 else _to_colmajor(x, want) for x in src]
             ptrs = [x.data_ptr() for x in u]
+            _order_after_torch(u[0])
         else:
             u = [np.asfortranarray(np.asarray(x).astype(str(self.prob.u0[0].dtype))) for x in src]
             ptrs = [x.ctypes.data for x in u]
@@ -633,8 +660,17 @@ def _contig_colmajor(x):
     return tuple(x.stride()) == _colmajor_strides(tuple(x.shape))
 
 
-def _to_colmajor(x):
-    y = device_empty(tuple(x.shape), str(x.dtype).replace("torch.", ""), x.device.index)
+def _order_after_torch(x):
+    """The library runs on its own non-blocking HIP stream: wait for work
+    that torch queued on the tensor's device (producers of u0, the layout
+    copy above) before the library reads the buffers."""
+    import torch
+
+    torch.cuda.current_stream(x.device).synchronize()
+
+
+def _to_colmajor(x, dtype=None):
+    y = device_empty(tuple(x.shape), dtype or str(x.dtype).replace("torch.", ""), x.device.index)
     y.copy_(x)
     return y
 

@@ -231,6 +231,12 @@ int gpe_sweep_last_ms(gpe_sweep* s, float* ms);
 int gpe_fno_gemm(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k,
                  const int64_t* strides, int64_t batch, float alpha, float beta, void* stream);
 
+/* gpe_fno_gemm with FP32 operands (FP32 matrix-core products): the
+ * transforms that band-limit a prediction, where BF16-rounded basis functions
+ * would add error at every wavenumber.  m / 64 <= 65535 for both. */
+int gpe_fno_gemm_f32(const float* A, const float* B, float* C, int64_t m, int64_t n, int64_t k,
+                     const int64_t* strides, int64_t batch, float alpha, float beta, void* stream);
+
 /* y = act(x + bias[ch]) over rows x cols values, ch = column (channels_last)
  * or (row mod C); act 1 = GELU (tanh form), 0 = identity; pre (may be NULL)
  * keeps x + bias for the backward. */

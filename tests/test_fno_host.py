@@ -22,7 +22,7 @@ class EmulatedOps:
     def _view(t, off, shape, strides):
         return torch.as_strided(t, shape, strides, t.storage_offset() + int(off))
 
-    def mm(self, A, B, C, m, n, k, a, b, c, batch=1, alpha=1.0, beta=0.0, offs=(0, 0, 0)):
+    def mm(self, A, B, C, m, n, k, a, b, c, batch=1, alpha=1.0, beta=0.0, offs=(0, 0, 0), fp32=False):
         Av = self._view(A, offs[0], (batch, m, k), (a[2], a[0], a[1]))
         Bv = self._view(B, offs[1], (batch, k, n), (b[2], b[0], b[1]))
         Cv = self._view(C, offs[2], (batch, m, n), (c[2], c[0], c[1]))
@@ -107,3 +107,21 @@ def test_oracle_spectral_layer_matches_fft():
                   bc=np.zeros((1, W)), Wp=np.eye(W), bp=np.zeros(W))
     o = fno_oracle.forward({k: torch.tensor(v) for k, v in params.items()}, torch.tensor(h), N, m, 1).numpy()
     assert np.abs(o - spec).max() <= 1e-12 * np.abs(spec).max()
+
+
+def test_band_limit_keeps_retained_modes_and_removes_the_rest(ggp):
+    N, m = 32, 6
+    net = _cpu_net(ggp, N, m, 4, 1, 4, 4)
+    x = np.arange(N)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    rng = np.random.default_rng(3)
+    inband = np.zeros((N, N))
+    for kx in range(-m + 1, m):
+        for ky in range(0, m):
+            a, b = rng.standard_normal(2)
+            inband += a * np.cos(2 * np.pi * (kx * X + ky * Y) / N) + b * np.sin(2 * np.pi * (kx * X + ky * Y) / N)
+    outband = np.cos(2 * np.pi * (3 * X + (m + 2) * Y) / N) + np.sin(2 * np.pi * ((m + 4) * X + 1 * Y) / N)
+    planes = torch.tensor(np.stack([inband, inband + outband])[None])
+    out = net.band_limit(planes).numpy()[0]
+    assert np.abs(out[0] - inband).max() <= 1e-12 * np.abs(inband).max()
+    assert np.abs(out[1] - inband).max() <= 1e-12 * np.abs(inband).max()
