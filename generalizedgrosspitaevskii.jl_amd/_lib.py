@@ -118,6 +118,15 @@ def load():
             raise RuntimeError(
                 f"{LIB_PATH} is not built; run `python generalizedgrosspitaevskii.jl_amd/build.py` "
                 "(or __graft_entry__.build()) first")
+        # torch ships its own libamdhip64.so.  Loaded after this library, it
+        # would start a second HIP runtime in the process (this library's from
+        # /opt/rocm), which then finds no devices and cannot use the other's
+        # pointers.  Loading torch first makes its runtime the one this
+        # library binds to (same soname).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -206,3 +206,28 @@ def refine(it, u0, *, tol=1e-8, check_every=50, max_steps=400000):
     """Plain split-step from u0 (e.g. a prediction) to the steady state on an
     existing iterator.  Returns (u, info) as StrangSplittingIterator.steady_state."""
     return it.steady_state(u0=u0, tol=tol, check_every=check_every, max_steps=max_steps)
+
+
+def solve_warm(it, ws, features, *, tol=1e-8, check_every=50, max_steps=400000, fallback=True):
+    """Steady state of the iterator's problem started from the surrogate's
+    band-limited prediction for `features` (one case, (1, C_in, N, N) on the
+    device), refined by plain split-step.  If the refinement does not reach
+    `tol` within max_steps and `fallback` is set, the case is rerun cold from
+    the problem's own u0.  Returns (u, info); info["start"] is "warm" or
+    "cold-fallback", info["ms"] the device time of everything run (inference
+    and, on fallback, both refinements)."""
+    torch = ws.torch
+    dt = torch.complex128 if "128" in str(it.prob.u0[0].dtype) else torch.complex64
+    ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ea.record()
+    pred = ws.predict_fields(features, dtype=dt)
+    eb.record()
+    torch.cuda.synchronize()
+    t_inf = ea.elapsed_time(eb)
+    u, info = it.steady_state(u0=tuple(x[0] for x in pred), tol=tol, check_every=check_every, max_steps=max_steps)
+    info = dict(info, start="warm", inference_ms=t_inf, ms=info["ms"] + t_inf)
+    if info["residual"] <= tol or not fallback:
+        return u, info
+    spent = info["ms"]
+    u, info = it.steady_state(tol=tol, check_every=check_every, max_steps=max_steps)
+    return u, dict(info, start="cold-fallback", inference_ms=t_inf, ms=info["ms"] + spent)

@@ -94,3 +94,23 @@ def test_create_without_device_reports_error(ggp):
     h = ctypes.c_void_p()
     assert lib.gpe_create(ctypes.byref(d), ctypes.byref(h)) == -3
     assert b"device" in lib.gpe_last_error()
+
+
+def test_one_hip_runtime_when_the_library_loads_first():
+    """Loading libgpe_hip.so before torch must not map a second HIP runtime
+    (torch bundles its own libamdhip64); checked in a fresh interpreter."""
+    import subprocess
+    import sys
+
+    code = (
+        "import sys, re; sys.path.insert(0, %r)\n"
+        "import __graft_entry__ as ge\n"
+        "g = ge.load_package()\n"
+        "from gpe_hip_amd import _lib\n"
+        "_lib.load()\n"
+        "maps = open('/proc/self/maps').read()\n"
+        "print(len(set(re.findall(r'\\S*libamdhip64\\S*', maps))))\n" % ROOT
+    )
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "1"
