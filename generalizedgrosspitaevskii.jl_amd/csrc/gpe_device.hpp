@@ -31,9 +31,24 @@ template <typename R>
 __device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b) {
   return mk<R>(a.re - b.re, a.im - b.im);
 }
+// Products of two terms are written with an explicit fused form.  Left to
+// floating-point contraction, x*y - z*w may fuse either product, and the
+// compiler can pick differently for the E values a thread holds once their
+// code is interleaved; the rounding of equal inputs then depends on the
+// value's position, which breaks exact symmetries of the field (a
+// homogeneous state stays homogeneous in exact arithmetic and must stay so
+// here, or split-step parametric modes near dt k^2/2 = n pi get a seed).
+template <typename R>
+__device__ __forceinline__ R fmx(R a, R b, R c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fmx<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
 template <typename R>
 __device__ __forceinline__ Cx<R> operator*(Cx<R> a, Cx<R> b) {
-  return mk<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+  return mk<R>(fmx<R>(a.re, b.re, -(a.im * b.im)), fmx<R>(a.re, b.im, a.im * b.re));
 }
 template <typename R>
 __device__ __forceinline__ Cx<R> scale(Cx<R> a, R s) {
@@ -50,7 +65,7 @@ __device__ __forceinline__ Cx<R> mul_mi(Cx<R> a) {
 }
 template <typename R>
 __device__ __forceinline__ R abs2(Cx<R> a) {
-  return a.re * a.re + a.im * a.im;
+  return fmx<R>(a.re, a.re, a.im * a.im);
 }
 
 // ---------------------------------------------------------------- butterflies

@@ -168,12 +168,17 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
       __syncthreads();
     }
   }
-  const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
+  // The branches below depend only on wave-uniform terms; each guards a
+  // straight loop over the E values so their arithmetic interleaves.
+  //
+  // e = cis(-delta G(u)) exp(-i delta V) from the pre-update fields
+  // (src/kernels.jl:40-44); has_e false means e == 1 exactly.
+  Cx<R> e[kE];
+  const bool has_e = pc.has_g || pt.pot_kind;
+  if (pc.has_g) {
+    R arg[kE], gim[kE];
 #pragma unroll
-  for (int m = 0; m < kE; ++m) {
-    Cx<R> e = mk<R>((R)1, (R)0);
-    // cis(-delta G(u)) from the pre-update fields (src/kernels.jl:40-44)
-    if (pc.has_g) {
+    for (int m = 0; m < kE; ++m) {
       R gr = pc.cr, gi = pc.ci;
       if (M > 1 && pt.g_coupled) {
 #pragma unroll
@@ -187,25 +192,77 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
         gr += gcr * a2j[0][m];
         gi += gci * a2j[0][m];
       }
-      R s, co;
-      fast_sincos(delta * gr, s, co);
-      // exp(-i delta (gr + i gi)) = exp(delta gi) (cos(delta gr) - i sin(delta gr))
-      if (pt.g_cplx) {
-        const R mag = exp(delta * gi);
-        e = mk<R>(mag * co, -mag * s);
-      } else {
-        e = mk<R>(co, -s);
-      }
+      arg[m] = delta * gr;
+      gim[m] = gi;
     }
-    if (pt.pot_kind) e = e * pc.pot[m];
-    // e (u + dt/2 F_now) + dt/2 F_next + noise   (src/kernels.jl:47-48)
-    Cx<R> w = v[m];
-    if (pc.pumped) w = w + pc.pump[m] * fnow;
-    w = e * w;
-    if (pc.pumped) w = w + pc.pump[m] * fnext;
-    if constexpr (NOISE) w = w + nz[m];
-    v[m] = w;
+    R s[kE], co[kE];
+    fast_sincos_n<R, kE>(arg, s, co);
+    // exp(-i delta (gr + i gi)) = exp(delta gi) (cos(delta gr) - i sin(delta gr))
+    if (pt.g_cplx) {
+#pragma unroll
+      for (int m = 0; m < kE; ++m) {
+        const R mag = exp(delta * gim[m]);
+        e[m] = mk<R>(mag * co[m], -mag * s[m]);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < kE; ++m) e[m] = mk<R>(co[m], -s[m]);
+    }
+    if (pt.pot_kind) {
+#pragma unroll
+      for (int m = 0; m < kE; ++m) e[m] = e[m] * pc.pot[m];
+    }
+  } else if (pt.pot_kind) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) e[m] = pc.pot[m];
   }
+  // e (u + dt/2 F_now) + dt/2 F_next + noise   (src/kernels.jl:47-48)
+  if (pc.pumped) {
+    const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
+    if (has_e) {
+#pragma unroll
+      for (int m = 0; m < kE; ++m) v[m] = e[m] * (v[m] + pc.pump[m] * fnow) + pc.pump[m] * fnext;
+    } else {
+#pragma unroll
+      for (int m = 0; m < kE; ++m) v[m] = (v[m] + pc.pump[m] * fnow) + pc.pump[m] * fnext;
+    }
+  } else if (has_e) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) v[m] = e[m] * v[m];
+  }
+  if constexpr (NOISE) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) v[m] = v[m] + nz[m];
+  }
+}
+
+// Two adjacent real-space half-steps with nothing between them (the one that
+// ends step s and the one that starts step s+1).  For a component whose
+// half-step only turns the phase -- no pump, noise or potential, G_c real and
+// a function of |u_c| alone -- the first leaves |u_c| unchanged, so the pair is
+// cis(-2 delta G(|u|^2)) u: one sine/cosine per value instead of two, equal to
+// the two half-steps up to rounding.  Other components take both half-steps.
+// The test is wave-uniform, and without coupling half_step has no barrier, so
+// the components' wavefronts may take different paths here.
+template <typename R, int N, int M, bool NOISE>
+__device__ __forceinline__ void half_step_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1, const HalfStep& h2,
+                                               const PointTerms& pt, const PointCache<R, M>& pc, int t, int T,
+                                               int64_t pbase, R* a2) {
+  const bool phase_only = pc.has_g && !pc.pumped && !pt.pot_kind && !pt.g_cplx && !(M > 1 && pt.g_coupled) &&
+                          (!NOISE || pc.eta == (R)0) && h1.delta == h2.delta;
+  if (!phase_only) {
+    half_step<R, N, M, NOISE>(v, c, h1, pt, pc, t, T, pbase, a2);
+    half_step<R, N, M, NOISE>(v, c, h2, pt, pc, t, T, pbase, a2);
+    return;
+  }
+  const R twod = (R)2 * (R)h1.delta;  // exact doubling of the half-step's delta
+  const R gcr = (M > 1 && c) ? pc.gr[M - 1] : pc.gr[0];
+  R arg[kE], s[kE], co[kE];
+#pragma unroll
+  for (int m = 0; m < kE; ++m) arg[m] = twod * (pc.cr + gcr * abs2(v[m]));
+  fast_sincos_n<R, kE>(arg, s, co);
+#pragma unroll
+  for (int m = 0; m < kE; ++m) v[m] = mk<R>(co[m], -s[m]) * v[m];
 }
 
 // exp(-i dt D(k)) / prod(N) of the k points a thread holds.  The table is
@@ -344,14 +401,18 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
     fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
     __syncthreads();
   }
-  if (a.flags & kEnd) half_step<R, N, M, NOISE>(v, c, a.h_end, a.pt, pc, t, T, line * (N / 2), a2);
   void* fr = c ? a.frame[M - 1] : a.frame[0];
-  if (fr != nullptr && valid) {
-    Cx<R>* dst = reinterpret_cast<Cx<R>*>(fr) + line * N;
+  if ((a.flags & kEnd) && (a.flags & kStart) && fr == nullptr) {
+    half_step_pair<R, N, M, NOISE>(v, c, a.h_end, a.h_start, a.pt, pc, t, T, line * (N / 2), a2);
+  } else {
+    if (a.flags & kEnd) half_step<R, N, M, NOISE>(v, c, a.h_end, a.pt, pc, t, T, line * (N / 2), a2);
+    if (fr != nullptr && valid) {
+      Cx<R>* dst = reinterpret_cast<Cx<R>*>(fr) + line * N;
 #pragma unroll
-    for (int m = 0; m < kE; ++m) dst[t + m * T] = v[m];
+      for (int m = 0; m < kE; ++m) dst[t + m * T] = v[m];
+    }
+    if (a.flags & kStart) half_step<R, N, M, NOISE>(v, c, a.h_start, a.pt, pc, t, T, line * (N / 2), a2);
   }
-  if (a.flags & kStart) half_step<R, N, M, NOISE>(v, c, a.h_start, a.pt, pc, t, T, line * (N / 2), a2);
   if (a.flags & kOutK) fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
   if (valid) {
 #pragma unroll

@@ -6,7 +6,14 @@
 // in FP32: truncation below 1e-19 and 2e-9 relative), and the quadrant
 // rotation by k mod 4.  Arguments outside the range where the split is exact
 // (|x| >= 1e9 in FP64, 1e5 in FP32) and non-finite ones take the library
-// routine out of line, so the inlined code stays a few dozen instructions.
+// routine out of line.
+//
+// fast_sincos_n evaluates E arguments at once: the polynomial path runs for
+// all of them without branches (so the E evaluations interleave), and one
+// fix-up region, entered only when some argument is wide, replaces those
+// results.  A per-value branch would split the code into E regions that the
+// compiler cannot interleave, leaving each value's ~20 dependent operations
+// exposed.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,13 +29,25 @@ struct SinCos {
 __host__ __device__ __noinline__ inline SinCos<double> sincos_wide(double x) { return {sin(x), cos(x)}; }
 __host__ __device__ __noinline__ inline SinCos<float> sincos_wide(float x) { return {sinf(x), cosf(x)}; }
 
-__host__ __device__ __forceinline__ void fast_sincos(double x, double& s, double& c) {
-  if (!(fabs(x) < 1e9)) {
-    const SinCos<double> w = sincos_wide(x);
-    s = w.s;
-    c = w.c;
-    return;
-  }
+__host__ __device__ __forceinline__ bool sincos_is_wide(double x) { return !(fabs(x) < 1e9); }
+__host__ __device__ __forceinline__ bool sincos_is_wide(float x) { return !(fabsf(x) < 1e5f); }
+
+// Rotation by the quadrant q = k mod 4, formed in floating point (exact for
+// the integers k of the narrow range; any value for wide arguments, whose
+// results are replaced).
+__host__ __device__ __forceinline__ void quadrant(double k, double sr, double cr, double& s, double& c) {
+  const double q = fma(-4.0, floor(k * 0.25), k);
+  s = (q == 0.0) ? sr : (q == 1.0) ? cr : (q == 2.0) ? -sr : -cr;
+  c = (q == 0.0) ? cr : (q == 1.0) ? -sr : (q == 2.0) ? -cr : sr;
+}
+__host__ __device__ __forceinline__ void quadrant(float k, float sr, float cr, float& s, float& c) {
+  const float q = fmaf(-4.0f, floorf(k * 0.25f), k);
+  s = (q == 0.0f) ? sr : (q == 1.0f) ? cr : (q == 2.0f) ? -sr : -cr;
+  c = (q == 0.0f) ? cr : (q == 1.0f) ? -sr : (q == 2.0f) ? -cr : sr;
+}
+
+// The narrow-range evaluation, branch-free.
+__host__ __device__ __forceinline__ void sincos_poly(double x, double& s, double& c) {
   const double k = rint(x * 0.6366197723675814);
   double r = fma(-k, 1.5707963267948966, x);
   r = fma(-k, 6.123233995736766e-17, r);
@@ -53,18 +72,10 @@ __host__ __device__ __forceinline__ void fast_sincos(double x, double& s, double
   q = fma(q, r2, 0.041666666666666664);     //  1/4!
   q = fma(q, r2, -0.5);                     // -1/2!
   const double cr = fma(q, r2, 1.0);
-  const int quad = (int)((long long)k & 3);
-  s = (quad == 0) ? sr : (quad == 1) ? cr : (quad == 2) ? -sr : -cr;
-  c = (quad == 0) ? cr : (quad == 1) ? -sr : (quad == 2) ? -cr : sr;
+  quadrant(k, sr, cr, s, c);
 }
 
-__host__ __device__ __forceinline__ void fast_sincos(float x, float& s, float& c) {
-  if (!(fabsf(x) < 1e5f)) {
-    const SinCos<float> w = sincos_wide(x);
-    s = w.s;
-    c = w.c;
-    return;
-  }
+__host__ __device__ __forceinline__ void sincos_poly(float x, float& s, float& c) {
   const float k = rintf(x * 0.63661975f);
   float r = fmaf(-k, 1.5707963705062866f, x);
   r = fmaf(-k, -4.371138828673793e-08f, r);
@@ -83,9 +94,36 @@ __host__ __device__ __forceinline__ void fast_sincos(float x, float& s, float& c
   q = fmaf(q, r2, 0.0416666679084301f);     //  1/4!
   q = fmaf(q, r2, -0.5f);                   // -1/2!
   const float cr = fmaf(q, r2, 1.0f);
-  const int quad = (int)k & 3;
-  s = (quad == 0) ? sr : (quad == 1) ? cr : (quad == 2) ? -sr : -cr;
-  c = (quad == 0) ? cr : (quad == 1) ? -sr : (quad == 2) ? -cr : sr;
+  quadrant(k, sr, cr, s, c);
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ void fast_sincos(R x, R& s, R& c) {
+  sincos_poly(x, s, c);
+  if (sincos_is_wide(x)) {
+    const SinCos<R> w = sincos_wide(x);
+    s = w.s;
+    c = w.c;
+  }
+}
+
+template <typename R, int E>
+__host__ __device__ __forceinline__ void fast_sincos_n(const R (&x)[E], R (&s)[E], R (&c)[E]) {
+  bool wide = false;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    sincos_poly(x[m], s[m], c[m]);
+    wide = wide || sincos_is_wide(x[m]);
+  }
+  if (wide) {
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      if (sincos_is_wide(x[m])) {
+        const SinCos<R> w = sincos_wide(x[m]);
+        s[m] = w.s;
+        c[m] = w.c;
+      }
+  }
 }
 
 }  // namespace gpe
