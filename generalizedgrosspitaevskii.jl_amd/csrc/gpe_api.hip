@@ -375,7 +375,8 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
   int rc = upload_into(&p, &h->dD_cap, compact.data(), compact.size() * sizeof(double));
   h->dD = (double*)p;
   if (rc) return rc;
-  rc = ensure_buf(&h->ed, &h->ed_cap, (size_t)e0 * e1 * per_out * h->csz);
+  // device records: four values per k for the two-component forms (see load_ed)
+  rc = ensure_buf(&h->ed, &h->ed_cap, (size_t)e0 * e1 * (dform >= 3 ? 4 : per_out) * h->csz);
   if (rc) return rc;
   h->disp_form = dform;
   h->ed_fold = fold;
@@ -527,6 +528,8 @@ static int prepare_tables(gpe_handle* h, double dt) {
     a.D = h->dD;
     a.out = h->ed;
     a.npts = h->ed_e0 * h->ed_e1;
+    a.e0 = h->ed_e0;
+    a.e1 = h->ed_e1;
     a.form = h->disp_form;
     a.M = h->M;
     a.dt = dt;
@@ -628,6 +631,7 @@ static EdDesc ed_desc(const gpe_handle* h) {
   d.n0 = h->n0;
   d.n1 = h->n1;
   d.e0 = h->ed_e0;
+  d.e1 = h->ed_e1;
   d.scale = 1.0 / (double)h->nsp;
   return d;
 }
@@ -1199,6 +1203,8 @@ extern "C" int gpe_sweep_solve(gpe_sweep* s, const void* u0, double t0, double t
     a.D = s->dD;
     a.out = s->ed;
     a.npts = s->total;
+    a.e0 = s->total;
+    a.e1 = 1;
     a.form = 1;
     a.M = 1;
     a.dt = rdt;

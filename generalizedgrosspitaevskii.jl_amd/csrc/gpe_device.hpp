@@ -189,12 +189,28 @@ __device__ __forceinline__ void load_twiddles(TwReg<R, N>& tr, int t, const Cx<R
   load_twiddles_pass<R, N, T, 1>(tr, t, tw, tws);
 }
 
+// Orders the LDS writes of a line's threads before their reads.  When the T
+// threads of every line lie inside one wavefront (LOCAL), a wavefront-scope
+// fence around a wave barrier suffices: a wavefront's LDS instructions execute
+// in order, so only the compiler must be kept from moving reads above writes.
+// Otherwise the whole workgroup synchronises.
+template <bool LOCAL>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (LOCAL) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham pass on M bundled lines followed (unless last) by an LDS
 // exchange into the cyclic layout.  `lds` points at this thread-group's lines;
 // line c of the bundle lives at lds + c*lstride (+ lds_pad(p)).  Two LDS
 // buffers (lds, lds + buf_off) alternate between passes so one barrier per
 // pass suffices.
-template <typename R, int N, int T, int M, bool INV, int NS, int BUF>
+template <typename R, int N, int T, int M, bool INV, int NS, int BUF, bool LOCAL>
 __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int lstride, int buf_off, int t,
                                               const TwReg<R, N>& tr) {
   constexpr int RADIX = PassRadix<N, NS>::value;
@@ -231,22 +247,24 @@ __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    line_sync<LOCAL>();
 #pragma unroll
     for (int c = 0; c < M; ++c)
 #pragma unroll
       for (int m = 0; m < kE; ++m) v[c][m] = buf[c * lstride + lds_pad(t + m * T)];
-    stockham_pass<R, N, T, M, INV, NS * RADIX, 1 - BUF>(v, lds, lstride, buf_off, t, tr);
+    stockham_pass<R, N, T, M, INV, NS * RADIX, 1 - BUF, LOCAL>(v, lds, lstride, buf_off, t, tr);
   }
 }
 
 // Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
-// of M bundled lines, input and output in the cyclic register layout.
-template <typename R, int N, int T, int M, bool INV>
+// of M bundled lines, input and output in the cyclic register layout.  LOCAL:
+// the caller guarantees each line's threads share one wavefront.
+template <typename R, int N, int T, int M, bool INV, bool LOCAL = false>
 __device__ __forceinline__ void fft_lines(Cx<R> (&v)[M][kE], Cx<R>* lds, int lstride, int buf_off, int t,
                                           const TwReg<R, N>& tr) {
   static_assert(N / T == kE, "E = N/T must be 8");
-  stockham_pass<R, N, T, M, INV, 1, 0>(v, lds, lstride, buf_off, t, tr);
+  static_assert(!LOCAL || (T <= 64 && 64 % T == 0), "a wave-local line must fit in one wavefront");
+  stockham_pass<R, N, T, M, INV, 1, 0, LOCAL>(v, lds, lstride, buf_off, t, tr);
 }
 
 // ------------------------------------------------------------------- Philox

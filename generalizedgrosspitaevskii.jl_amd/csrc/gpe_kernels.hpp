@@ -267,15 +267,18 @@ __device__ __forceinline__ void half_step_pair(Cx<R> (&v)[kE], int c, const Half
 
 // exp(-i dt D(k)) / prod(N) of the k points a thread holds.  The table is
 // indexed by (k0', k1') where k_d' = min(k_d, N_d - k_d) when D is even in
-// k_d (fold bit d), giving (N_d/2+1) rows instead of N_d.  Entry forms:
-// 1 scalar, 2 diagonal (M values), 3 full 2x2 (column-major), 4 symmetric
-// 2x2 stored as (a, b, d) for [a b; b d].
+// k_d (fold bit d), giving (N_d/2+1) rows instead of N_d, and stored with k1'
+// fastest: the column pass reads a column's entries contiguously.  Entry
+// forms: 1 scalar, 2 diagonal (M values), and for two components a record of
+// four values [E00, E01, E10, E11] (3 full 2x2; 4 symmetric, [a, b, b, d]):
+// component 0 takes its pair (own, other) from slots 0-1 and component 1 its
+// pair (other, own) from slots 2-3, each as one aligned load.
 struct EdDesc {
   int form;       // 0: no dispersion, multiply by `scale` only
   int fold;       // bit 0: dim 0 folded, bit 1: dim 1 folded
   const void* tab;
   int64_t n0, n1; // grid extents
-  int64_t e0;     // table row length along dim 0
+  int64_t e0, e1; // table extents along dim 0 and dim 1 (folded)
   double scale;
 };
 
@@ -290,30 +293,32 @@ struct EdVals {
 __device__ __forceinline__ int64_t ed_index(const EdDesc& d, int64_t k0, int64_t k1) {
   if (d.fold & 1) k0 = (k0 <= d.n0 - k0) ? k0 : d.n0 - k0;
   if (d.fold & 2) k1 = (k1 <= d.n1 - k1) ? k1 : d.n1 - k1;
-  return k0 + d.e0 * k1;
+  return k1 + d.e1 * k0;
 }
+
+template <typename R>
+struct alignas(2 * sizeof(Cx<R>)) CxPair {
+  Cx<R> a, b;
+};
 
 // k points (k0, k1) = (a0 + s0 m, a1 + s1 m), m = 0..E-1.
 template <typename R, int M>
 __device__ __forceinline__ void load_ed(EdVals<R>& ev, const EdDesc& d, int c, int64_t a0, int64_t s0, int64_t a1,
                                         int64_t s1) {
   if (d.form == 0) return;
-  int per, io, ix;
-  if (d.form == 1) {
-    per = 1; io = 0; ix = 0;
-  } else if (d.form == 2) {
-    per = M; io = c; ix = 0;
-  } else if (d.form == 3) {
-    per = 4; io = c ? 3 : 0; ix = c ? 1 : 2;
-  } else {
-    per = 3; io = c ? 2 : 0; ix = 1;
-  }
+  if (M > 1 && d.form >= 3) {
+    const CxPair<R>* rec = reinterpret_cast<const CxPair<R>*>(d.tab);
 #pragma unroll
-  for (int m = 0; m < kE; ++m) {
-    const int64_t idx = ed_index(d, a0 + s0 * m, a1 + s1 * m) * per;
-    ev.own[m] = ldc<R>(d.tab, idx + io);
-    if (M > 1 && d.form >= 3) ev.other[m] = ldc<R>(d.tab, idx + ix);
+    for (int m = 0; m < kE; ++m) {
+      const CxPair<R> p = rec[2 * ed_index(d, a0 + s0 * m, a1 + s1 * m) + c];
+      ev.own[m] = c ? p.b : p.a;
+      ev.other[m] = c ? p.a : p.b;
+    }
+    return;
   }
+  const int per = (d.form == 2) ? M : 1, io = (d.form == 2) ? c : 0;
+#pragma unroll
+  for (int m = 0; m < kE; ++m) ev.own[m] = ldc<R>(d.tab, ed_index(d, a0 + s0 * m, a1 + s1 * m) * per + io);
 }
 
 // Apply row c of exp(-i dt D) / prod(N).  For the 2x2 forms the partner
@@ -397,9 +402,13 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
   Cx<R>* my = lds + (c * W + l) * LS;
   const int boff = M * W * LS;
   R* a2 = reinterpret_cast<R*>(lds) + (size_t)l * M * N;
+  // Lines are T consecutive threads, so for T <= 64 a line is wave-local.
+  constexpr bool LOCAL = T <= 64;
   if (a.flags & kInK) {
-    fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
-    __syncthreads();
+    fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
+    // the coupled half-step reuses buffer 0 for |u_j|^2 (it starts with a
+    // workgroup barrier of its own); no other LDS use follows
+    if (M > 1 && a.pt.g_coupled) __syncthreads();
   }
   void* fr = c ? a.frame[M - 1] : a.frame[0];
   if ((a.flags & kEnd) && (a.flags & kStart) && fr == nullptr) {
@@ -413,7 +422,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
     }
     if (a.flags & kStart) half_step<R, N, M, NOISE>(v, c, a.h_start, a.pt, pc, t, T, line * (N / 2), a2);
   }
-  if (a.flags & kOutK) fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
+  if (a.flags & kOutK) fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
   if (valid) {
 #pragma unroll
     for (int m = 0; m < kE; ++m) uc[t + m * T] = v[m];
@@ -469,10 +478,12 @@ __global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
   Cx<R>* my = lds + (c * C + cx) * LS;
   const Cx<R>* partner = lds + ((M - 1 - c) * C + cx) * LS;
   const int boff = M * C * LS;
-  fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
+  // a component's C columns interleave over C*T consecutive threads
+  constexpr bool LOCAL = C * T <= 64;
+  fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
   apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
-  __syncthreads();
-  fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
+  line_sync<LOCAL>();
+  fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
 #pragma unroll
   for (int m = 0; m < kE; ++m) uc[(int64_t)(t + m * T) * nx] = v[m];
 }
@@ -496,6 +507,7 @@ struct Step1DArgs {
 template <typename R, int N, int M, int W, bool NOISE>
 __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) {
   constexpr int T = N / kE;
+  constexpr bool LOCAL = T <= 64;  // lines are T consecutive threads
   constexpr int LS = lds_line_stride(N);
   extern __shared__ unsigned char smem_raw[];
   Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
@@ -538,17 +550,17 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
     h.fnext[1] = q * en[1][1];
     h.call = a.call0 + 2 * (uint64_t)s;
     half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line * (N / 2), a2);
-    fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
+    fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
     apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
-    __syncthreads();
-    fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
+    line_sync<LOCAL>();
+    fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
     h.fnow[0] = q * en[1][0];
     h.fnow[1] = q * en[1][1];
     h.fnext[0] = q * en[2][0];
     h.fnext[1] = q * en[2][1];
     h.call += 1;
     half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line * (N / 2), a2);
-    __syncthreads();
+    line_sync<LOCAL>();
   }
   if (valid) {
 #pragma unroll
@@ -569,9 +581,10 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
 // A = -i dt D, tau = tr(A)/2, s^2 = ((a-d)/2)^2 + bc; Taylor series for |s|
 // small.
 struct EdArgs {
-  const double* D;  // complex FP64 table, npts * per entries
-  void* out;
+  const double* D;  // complex FP64 table, npts * per entries, k0 fastest (compact order)
+  void* out;        // device table, k1 fastest (ed_index order)
   int64_t npts;
+  int64_t e0, e1;   // compact extents: npts = e0 * e1
   int form;         // 1 scalar, 2 diag, 3 full 2x2, 4 symmetric 2x2 (a, b, d)
   int M;
   double dt;
@@ -638,13 +651,14 @@ __global__ void ed_table_kernel(EdArgs a) {
   if (k >= a.npts) return;
   Cx<R>* out = reinterpret_cast<Cx<R>*>(a.out);
   const Cx<double>* D = reinterpret_cast<const Cx<double>*>(a.D);
+  const int64_t o = k / a.e0 + a.e1 * (k % a.e0);  // transposed position
   if (a.form == 1 || a.form == 2) {
     const int per = (a.form == 1) ? 1 : a.M;
     for (int i = 0; i < per; ++i) {
       const Cx<double> z = D[k * per + i];
       // exp(-i dt z)
       const Cx<double> e = cexp_d(mk<double>(a.dt * z.im, -a.dt * z.re));
-      out[k * per + i] = mk<R>((R)(e.re * a.scale), (R)(e.im * a.scale));
+      out[o * per + i] = mk<R>((R)(e.re * a.scale), (R)(e.im * a.scale));
     }
   } else if (a.form == 4) {
     // symmetric 2x2 stored as (a, b, d); exp of a symmetric matrix is symmetric
@@ -655,9 +669,10 @@ __global__ void ed_table_kernel(EdArgs a) {
       A[i] = mk<double>(a.dt * z.im, -a.dt * z.re);
     }
     expm2x2_d(A, E);
-    const int dst[3] = {0, 1, 3};
-    for (int i = 0; i < 3; ++i)
-      out[k * 3 + i] = mk<R>((R)(E[dst[i]].re * a.scale), (R)(E[dst[i]].im * a.scale));
+    // record [E00, E01, E10, E11] = [a, b, b, d]
+    const int src4[4] = {0, 2, 1, 3};
+    for (int i = 0; i < 4; ++i)
+      out[o * 4 + i] = mk<R>((R)(E[src4[i]].re * a.scale), (R)(E[src4[i]].im * a.scale));
   } else {
     Cx<double> A[4], E[4];
     for (int i = 0; i < 4; ++i) {
@@ -665,7 +680,10 @@ __global__ void ed_table_kernel(EdArgs a) {
       A[i] = mk<double>(a.dt * z.im, -a.dt * z.re);
     }
     expm2x2_d(A, E);
-    for (int i = 0; i < 4; ++i) out[k * 4 + i] = mk<R>((R)(E[i].re * a.scale), (R)(E[i].im * a.scale));
+    // E column-major [E00, E10, E01, E11] -> record [E00, E01, E10, E11]
+    const int src4[4] = {0, 2, 1, 3};
+    for (int i = 0; i < 4; ++i)
+      out[o * 4 + i] = mk<R>((R)(E[src4[i]].re * a.scale), (R)(E[src4[i]].im * a.scale));
   }
 }
 
@@ -730,6 +748,7 @@ struct SweepArgs {
 template <typename R, int N, int W, bool NOISE>
 __global__ void __launch_bounds__(W * (N / kE)) sweep1d_kernel(SweepArgs a) {
   constexpr int T = N / kE;
+  constexpr bool LOCAL = T <= 64;  // instances are T consecutive threads
   constexpr int LS = lds_line_stride(N);
   extern __shared__ unsigned char smem_raw[];
   Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
@@ -790,17 +809,17 @@ __global__ void __launch_bounds__(W * (N / kE)) sweep1d_kernel(SweepArgs a) {
     h.fnext[1] = q * en[1][1];
     h.call = a.call0 + 2 * (uint64_t)s;
     half_step<R, N, 1, NOISE>(v, 0, h, pt, pc, t, T, pbase, nullptr);
-    fft_lines<R, N, T, 1, false>(as_bundle(v), my, 0, boff, t, tr);
+    fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
     apply_ed<R, 1>(v, ed, ev, t, T, my, my);
-    __syncthreads();
-    fft_lines<R, N, T, 1, true>(as_bundle(v), my, 0, boff, t, tr);
+    line_sync<LOCAL>();
+    fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
     h.fnow[0] = q * en[1][0];
     h.fnow[1] = q * en[1][1];
     h.fnext[0] = q * en[2][0];
     h.fnext[1] = q * en[2][1];
     h.call += 1;
     half_step<R, N, 1, NOISE>(v, 0, h, pt, pc, t, T, pbase, nullptr);
-    __syncthreads();
+    line_sync<LOCAL>();
   }
   if (valid) {
 #pragma unroll
