@@ -158,6 +158,18 @@ __host__ __device__ constexpr int tw_count_before(int N, int NS) {
 }
 __host__ __device__ constexpr int tw_count(int N) { return tw_count_before(N, N) > 0 ? tw_count_before(N, N) : 1; }
 
+// LDS exchanges of one length-N transform (passes - 1).  Pass p writes buffer
+// (BUF0 + p) mod 2, so the last exchange uses buffer (BUF0 + exchanges - 1) mod 2.
+__host__ __device__ constexpr int fft_exchanges(int N) {
+  int ns = 1, p = 0;
+  while (ns < N) {
+    const int rem = N / ns;
+    ns *= (rem % 8 == 0) ? 8 : rem;
+    ++p;
+  }
+  return p - 1;
+}
+
 // Per-thread twiddle registers of one transform length, loaded once (at
 // kernel start, together with the field) from the FP64-accurate table
 // tw[m] = exp(-2 pi i m / NT), NT = N * tws.  The inverse transform uses the
@@ -259,12 +271,12 @@ __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int
 // Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
 // of M bundled lines, input and output in the cyclic register layout.  LOCAL:
 // the caller guarantees each line's threads share one wavefront.
-template <typename R, int N, int T, int M, bool INV, bool LOCAL = false>
+template <typename R, int N, int T, int M, bool INV, bool LOCAL = false, int BUF0 = 0>
 __device__ __forceinline__ void fft_lines(Cx<R> (&v)[M][kE], Cx<R>* lds, int lstride, int buf_off, int t,
                                           const TwReg<R, N>& tr) {
   static_assert(N / T == kE, "E = N/T must be 8");
   static_assert(!LOCAL || (T <= 64 && 64 % T == 0), "a wave-local line must fit in one wavefront");
-  stockham_pass<R, N, T, M, INV, 1, 0, LOCAL>(v, lds, lstride, buf_off, t, tr);
+  stockham_pass<R, N, T, M, INV, 1, BUF0, LOCAL>(v, lds, lstride, buf_off, t, tr);
 }
 
 // ------------------------------------------------------------------- Philox

@@ -480,10 +480,30 @@ __global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
   const int boff = M * C * LS;
   // a component's C columns interleave over C*T consecutive threads
   constexpr bool LOCAL = C * T <= 64;
+  // buffer read by the forward transform's last exchange; the other one is
+  // free once that transform ends, so the components exchange through it and
+  // the inverse transform starts in this one (no barrier between them)
+  constexpr int LAST = (fft_exchanges(N) - 1) & 1;
   fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
-  apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
-  line_sync<LOCAL>();
-  fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
+  if (M > 1 && a.ed.form >= 3) {
+    Cx<R>* mine = my + (1 - LAST) * boff;
+    const Cx<R>* theirs = partner + (1 - LAST) * boff;
+#pragma unroll
+    for (int m = 0; m < kE; ++m) mine[lds_pad(t + m * T)] = v[m];
+    __syncthreads();
+    Cx<R> p[kE];
+#pragma unroll
+    for (int m = 0; m < kE; ++m) p[m] = theirs[lds_pad(t + m * T)];
+    // a wave-local inverse transform would not wait for the partner's reads
+    // before writing buffer 1 - LAST (its second pass)
+    if constexpr (LOCAL) __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kE; ++m) v[m] = ev.own[m] * v[m] + ev.other[m] * p[m];
+  } else {
+    apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
+    line_sync<LOCAL>();
+  }
+  fft_lines<R, N, T, 1, true, LOCAL, LAST>(as_bundle(v), my, 0, boff, t, tr);
 #pragma unroll
   for (int m = 0; m < kE; ++m) uc[(int64_t)(t + m * T) * nx] = v[m];
 }
