@@ -150,8 +150,6 @@ template <typename R, int N, int M, bool NOISE>
 __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep& hs, const PointTerms& pt,
                                           const PointCache<R, M>& pc, int t, int T, int64_t pbase, R* a2) {
   const R delta = (R)hs.delta;
-  Cx<R> nz[kE];
-  if constexpr (NOISE) noise_terms<R, N>(nz, hs, pt, c, pc.eta, t, T, pbase);
   R a2j[M][kE];
   if (pt.gmask) {
 #pragma unroll
@@ -167,6 +165,42 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
         for (int m = 0; m < kE; ++m) a2j[j][m] = a2[j * N + t + m * T];
       __syncthreads();
     }
+  }
+  if constexpr (NOISE) {
+    // Stochastic kernels are bound by the Philox/Box-Muller arithmetic and
+    // need their occupancy: one compact loop over the values (fewer live
+    // registers and less code than the interleaved form below).
+    Cx<R> nz[kE];
+    noise_terms<R, N>(nz, hs, pt, c, pc.eta, t, T, pbase);
+    const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
+#pragma unroll
+    for (int m = 0; m < kE; ++m) {
+      Cx<R> e = mk<R>((R)1, (R)0);
+      if (pc.has_g) {
+        R gr = pc.cr, gi = pc.ci;
+        if (M > 1 && pt.g_coupled) {
+#pragma unroll
+          for (int j = 0; j < M; ++j) {
+            gr += pc.gr[j] * a2j[j][m];
+            gi += pc.gi[j] * a2j[j][m];
+          }
+        } else {
+          gr += ((M > 1 && c) ? pc.gr[M - 1] : pc.gr[0]) * a2j[0][m];
+          gi += ((M > 1 && c) ? pc.gi[M - 1] : pc.gi[0]) * a2j[0][m];
+        }
+        R sn, co;
+        fast_sincos(delta * gr, sn, co);
+        const R mag = pt.g_cplx ? (R)exp(delta * gi) : (R)1;
+        e = mk<R>(mag * co, -mag * sn);
+      }
+      if (pt.pot_kind) e = e * pc.pot[m];
+      Cx<R> w = v[m];
+      if (pc.pumped) w = w + pc.pump[m] * fnow;
+      w = e * w;
+      if (pc.pumped) w = w + pc.pump[m] * fnext;
+      v[m] = w + nz[m];
+    }
+    return;
   }
   // The branches below depend only on wave-uniform terms; each guards a
   // straight loop over the E values so their arithmetic interleaves.
@@ -230,10 +264,6 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
 #pragma unroll
     for (int m = 0; m < kE; ++m) v[m] = e[m] * v[m];
   }
-  if constexpr (NOISE) {
-#pragma unroll
-    for (int m = 0; m < kE; ++m) v[m] = v[m] + nz[m];
-  }
 }
 
 // Two adjacent real-space half-steps with nothing between them (the one that
@@ -241,20 +271,17 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
 // half-step only turns the phase -- no pump, noise or potential, G_c real and
 // a function of |u_c| alone -- the first leaves |u_c| unchanged, so the pair is
 // cis(-2 delta G(|u|^2)) u: one sine/cosine per value instead of two, equal to
-// the two half-steps up to rounding.  Other components take both half-steps.
-// The test is wave-uniform, and without coupling half_step has no barrier, so
-// the components' wavefronts may take different paths here.
-template <typename R, int N, int M, bool NOISE>
-__device__ __forceinline__ void half_step_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1, const HalfStep& h2,
-                                               const PointTerms& pt, const PointCache<R, M>& pc, int t, int T,
-                                               int64_t pbase, R* a2) {
-  const bool phase_only = pc.has_g && !pc.pumped && !pt.pot_kind && !pt.g_cplx && !(M > 1 && pt.g_coupled) &&
-                          (!NOISE || pc.eta == (R)0) && h1.delta == h2.delta;
-  if (!phase_only) {
-    half_step<R, N, M, NOISE>(v, c, h1, pt, pc, t, T, pbase, a2);
-    half_step<R, N, M, NOISE>(v, c, h2, pt, pc, t, T, pbase, a2);
-    return;
-  }
+// the two half-steps up to rounding.  The test is wave-uniform, and without
+// coupling half_step has no barrier, so the components' wavefronts may take
+// different paths.
+template <typename R, int M>
+__device__ __forceinline__ bool phase_only(const PointTerms& pt, const PointCache<R, M>& pc, const HalfStep& h1,
+                                           const HalfStep& h2) {
+  return pc.has_g && !pc.pumped && !pt.pot_kind && !pt.g_cplx && !(M > 1 && pt.g_coupled) && h1.delta == h2.delta;
+}
+
+template <typename R, int M>
+__device__ __forceinline__ void phase_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1, const PointCache<R, M>& pc) {
   const R twod = (R)2 * (R)h1.delta;  // exact doubling of the half-step's delta
   const R gcr = (M > 1 && c) ? pc.gr[M - 1] : pc.gr[0];
   R arg[kE], s[kE], co[kE];
@@ -411,8 +438,12 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
     if (M > 1 && a.pt.g_coupled) __syncthreads();
   }
   void* fr = c ? a.frame[M - 1] : a.frame[0];
-  if ((a.flags & kEnd) && (a.flags & kStart) && fr == nullptr) {
-    half_step_pair<R, N, M, NOISE>(v, c, a.h_end, a.h_start, a.pt, pc, t, T, line * (N / 2), a2);
+  // stochastic runs keep both half-steps: the noise acts on the components
+  bool pair = false;
+  if constexpr (!NOISE)
+    pair = (a.flags & kEnd) && (a.flags & kStart) && fr == nullptr && phase_only<R, M>(a.pt, pc, a.h_end, a.h_start);
+  if (pair) {
+    phase_pair<R, M>(v, c, a.h_end, pc);
   } else {
     if (a.flags & kEnd) half_step<R, N, M, NOISE>(v, c, a.h_end, a.pt, pc, t, T, line * (N / 2), a2);
     if (fr != nullptr && valid) {
