@@ -437,6 +437,66 @@ def warmstart_run(g, torch, args):
     print(json.dumps(out))
 
 
+def fno_bench(torch, batch=16, reps=10):
+    """Throughput of the warm-start network's kernels alone: the 256x256 FNO
+    (width 32, 16 modes, 4 layers, random weights) forward on `batch` cases
+    and one training step (forward, backward, Adam) on the same batch.
+    FLOPs are counted from the products issued (2 m n k per product)."""
+    from gpe_hip_amd.fno import FNO
+
+    net = FNO(256, in_ch=4, out_ch=4, width=32, modes=16, layers=4, seed=0)
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(0x5EED0005)
+    x = torch.randn(batch, 4, 256, 256, device="cuda:0", generator=gen)
+    t = torch.randn(batch, 4, 256, 256, device="cuda:0", generator=gen)
+    out = {}
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        f0, b0 = net.ops.flops, net.ops.bytes
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / n
+        fl = (net.ops.flops - f0) / n
+        by = (net.ops.bytes - b0) / n
+        return dict(ms=ms, gflop=fl / 1e9, tflops=fl / ms / 1e9, operand_gb=by / 1e9, gbs=by / ms / 1e6,
+                    flop_per_byte=fl / by)
+
+    out["forward_one_case"] = timed(lambda: net.forward(x[:1]), reps)
+    out["forward_batch"] = timed(lambda: net.forward(x), reps)
+
+    def train():
+        net.zero_grad()
+        net.loss_and_grad(x, t)
+        net.adam_step(1e-4)
+
+    out["train_step_batch"] = timed(train, max(2, reps // 2))
+    fb = out["forward_batch"]
+    return {
+        "metric": "FNO forward throughput (warm-start network, 256x256)",
+        "value": fb["tflops"],
+        "unit": "TFLOP/s",
+        "note": "about 8 flop per operand byte, far below the bf16 ridge (~312): the forward is bound by memory; "
+                "the roofline object uses operand bytes (each operand read once, outputs written, accumulated "
+                "outputs also read)",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "bf16 x bf16 -> f32",
+        "data": "synthetic (random weights and inputs)",
+        "config": {"workload": f"FNO width 32, 16 modes, 4 layers, {batch} cases of 256x256x4"},
+        "roofline": {"bound": "hbm", "kernel": "gemm_kernel<bf16> and bias/GELU (whole forward)",
+                     "achieved": fb["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": fb["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "mfma_frac": fb["tflops"] / BF16_DENSE_PEAK_TFS},
+        **out,
+    }
+
+
 def cpu_baseline(seconds=12.0):
     """FP64 numpy oracle (one host core) on a bounded sample of the same
     workload: as many steps as fit in about `seconds`, from the state after a
@@ -489,6 +549,7 @@ def main():
     ap.add_argument("--ws-save", default=None, help="write the trained surrogate (.npz)")
     ap.add_argument("--ws-load", default=None, help="skip data and training, load a surrogate (.npz)")
     ap.add_argument("--ws-diag", type=int, default=0, help="controlled-start comparison on this many test cases")
+    ap.add_argument("--fno-bench", action="store_true", help="run only the warm-start network's throughput")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (PMC calibration)")
     args = ap.parse_args()
@@ -512,6 +573,9 @@ def main():
     g = ge.load_package()
     if args.warmstart:
         warmstart_run(g, torch, args)
+        return
+    if args.fno_bench:
+        print(json.dumps(fno_bench(torch)))
         return
     runs = {}
     for dtype in ([args.dtype] + [d for d in ("complex64", "complex128") if d != args.dtype]):

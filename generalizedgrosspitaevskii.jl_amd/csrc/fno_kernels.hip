@@ -35,7 +35,7 @@ __device__ __forceinline__ short to_bf16(float f) {
   return (short)(u >> 16);
 }
 
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TM = 64, TN = 64, TK = 32;
 
 // Element (i, j) of an operand sits at base + i * rs + j * cs; batch b adds
 // b * bs.  Transposed and strided views (channels, modes) are stride choices.
@@ -50,8 +50,11 @@ struct GemmArgs {
 };
 
 // 256 threads = 4 wavefronts; the workgroup computes a 64 x 64 tile of C,
-// each wavefront a 32 x 32 quarter as 2 x 2 MFMA 16x16 blocks.  A and B
-// tiles of depth 16 are staged through LDS (BF16 or FP32 per kBF16).
+// each wavefront a 32 x 32 quarter as 2 x 2 MFMA 16x16 blocks.  The k loop
+// runs over tiles of depth 32 that are double-buffered in LDS (BF16 or FP32
+// per kBF16): while the matrix cores work on tile t, the next tile's values
+// are already in flight into registers, and one barrier per tile separates
+// the buffers.
 //
 // MFMA 16x16x16 BF16 operand layout (lane l of the wavefront):
 //   A fragment: row l % 16, k = 4 (l / 16) .. 4 (l / 16) + 3
@@ -61,8 +64,8 @@ struct GemmArgs {
 template <bool kBF16>
 struct GemmLds {
   using T = typename std::conditional<kBF16, short, float>::type;
-  T As[TM][TK + 4];
-  T Bs[TK][TN + 4];
+  T As[2][TM][TK + 4];
+  T Bs[2][TK][TN + 4];
 };
 
 template <bool kBF16>
@@ -72,6 +75,9 @@ __device__ __forceinline__ typename GemmLds<kBF16>::T lds_value(float v) {
   else
     return v;
 }
+
+constexpr int kPerThread = TM * TK / 256;  // A (and B) values each thread stages per tile
+static_assert(TM * TK == TK * TN, "square staging");
 
 template <bool kBF16>
 __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
@@ -87,55 +93,71 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
   floatx4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int64_t k0 = 0; k0 < g.k; k0 += TK) {
-    // stage A: 64 x 16 values, 4 per thread
-    for (int e = tid; e < TM * TK; e += 256) {
-      const int r = e / TK, kk = e % TK;
-      const int64_t gi = m0 + r, gk = k0 + kk;
-      float v = 0.f;
-      if (gi < g.m && gk < g.k) v = A[gi * g.ars + gk * g.acs];
-      L.As[r][kk] = lds_value<kBF16>(v);
+  // staging map: A value e -> (row e / TK, k e % TK), B value e -> (k e / TN, col e % TN)
+  float ra[kPerThread], rb[kPerThread];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const int e = tid + q * 256;
+      const int64_t gi = m0 + e / TK, gka = k0 + e % TK;
+      ra[q] = (gi < g.m && gka < g.k) ? A[gi * g.ars + gka * g.acs] : 0.f;
+      const int64_t gkb = k0 + e / TN, gj = n0 + e % TN;
+      rb[q] = (gkb < g.k && gj < g.n) ? B[gkb * g.brs + gj * g.bcs] : 0.f;
     }
-    // stage B: 16 x 64 values
-    for (int e = tid; e < TK * TN; e += 256) {
-      const int kk = e / TN, c = e % TN;
-      const int64_t gk = k0 + kk, gj = n0 + c;
-      float v = 0.f;
-      if (gk < g.k && gj < g.n) v = B[gk * g.brs + gj * g.bcs];
-      L.Bs[kk][c] = lds_value<kBF16>(v);
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const int e = tid + q * 256;
+      L.As[buf][e / TK][e % TK] = lds_value<kBF16>(ra[q]);
+      L.Bs[buf][e / TN][e % TN] = lds_value<kBF16>(rb[q]);
     }
-    __syncthreads();
+  };
+  const int64_t nt = (g.k + TK - 1) / TK;
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nt; ++kt) {
+    const int buf = (int)(kt & 1);
+    if (kt + 1 < nt) fetch((kt + 1) * TK);  // in flight while the tile below is consumed
     if constexpr (kBF16) {
-      const int kb = 4 * (lane / 16);
-      for (int i = 0; i < 2; ++i) {
-        bf16x4 a;
-        const int r = wm + 16 * i + lane % 16;
-        a[0] = L.As[r][kb];
-        a[1] = L.As[r][kb + 1];
-        a[2] = L.As[r][kb + 2];
-        a[3] = L.As[r][kb + 3];
-        for (int j = 0; j < 2; ++j) {
-          bf16x4 b;
-          const int c = wn + 16 * j + lane % 16;
-          b[0] = L.Bs[kb][c];
-          b[1] = L.Bs[kb + 1][c];
-          b[2] = L.Bs[kb + 2][c];
-          b[3] = L.Bs[kb + 3][c];
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int ks = 0; ks < TK; ks += 16) {
+        const int kb = ks + 4 * (lane / 16);
+        for (int i = 0; i < 2; ++i) {
+          bf16x4 a;
+          const int r = wm + 16 * i + lane % 16;
+          a[0] = L.As[buf][r][kb];
+          a[1] = L.As[buf][r][kb + 1];
+          a[2] = L.As[buf][r][kb + 2];
+          a[3] = L.As[buf][r][kb + 3];
+          for (int j = 0; j < 2; ++j) {
+            bf16x4 b;
+            const int c = wn + 16 * j + lane % 16;
+            b[0] = L.Bs[buf][kb][c];
+            b[1] = L.Bs[buf][kb + 1][c];
+            b[2] = L.Bs[buf][kb + 2][c];
+            b[3] = L.Bs[buf][kb + 3][c];
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+          }
         }
       }
     } else {
+#pragma unroll
       for (int ks = 0; ks < TK; ks += 4) {
         const int kk = ks + lane / 16;
         for (int i = 0; i < 2; ++i) {
-          const float a = L.As[wm + 16 * i + lane % 16][kk];
+          const float a = L.As[buf][wm + 16 * i + lane % 16][kk];
           for (int j = 0; j < 2; ++j) {
-            const float b = L.Bs[kk][wn + 16 * j + lane % 16];
+            const float b = L.Bs[buf][kk][wn + 16 * j + lane % 16];
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
           }
         }
       }
     }
+    // the other buffer was last read in the previous tile, before the
+    // barrier that ended it
+    if (kt + 1 < nt) stage(buf ^ 1);
     __syncthreads();
   }
   for (int i = 0; i < 2; ++i)

@@ -42,25 +42,31 @@ class _Ops:
         self.lib = L.load()
         self.stream = stream
         self.flops = 0  # multiply-adds x 2 issued through mm (roofline accounting)
+        self.bytes = 0  # operand bytes: each operand read once, C written (and read when beta != 0)
 
     def mm(self, A, B, C, m, n, k, a, b, c, batch=1, alpha=1.0, beta=0.0, offs=(0, 0, 0), fp32=False):
         """C = alpha A B + beta C; a/b/c = (row stride, col stride, batch stride);
         BF16 operands unless fp32."""
         st = (ctypes.c_int64 * 9)(a[0], a[1], b[0], b[1], c[0], c[1], a[2], b[2], c[2])
         self.flops += 2 * m * n * k * batch
+        self.bytes += 4 * (m * k * (batch if a[2] else 1) + k * n * (batch if b[2] else 1)
+                           + m * n * batch * (2 if beta != 0.0 else 1))
         fn = self.lib.gpe_fno_gemm_f32 if fp32 else self.lib.gpe_fno_gemm
         L.check(fn(_ptr(A, offs[0]), _ptr(B, offs[1]), _ptr(C, offs[2]), m, n, k, st, batch, float(alpha),
                    float(beta), self.stream))
 
     def bias_act(self, x, bias, pre, y, rows, cols, C, act):
+        self.bytes += 4 * rows * cols * (3 if pre is not None else 2)
         L.check(self.lib.gpe_fno_bias_act(_ptr(x), None if bias is None else _ptr(bias),
                                           None if pre is None else _ptr(pre), _ptr(y), rows, cols, C, 0, int(act),
                                           self.stream))
 
     def act_backward(self, pre, dy, dx, n):
+        self.bytes += 12 * n
         L.check(self.lib.gpe_fno_act_backward(_ptr(pre), _ptr(dy), _ptr(dx), n, self.stream))
 
     def axpby(self, x, y, a, b, n):
+        self.bytes += 12 * n
         L.check(self.lib.gpe_fno_axpby(_ptr(x), _ptr(y), float(a), float(b), n, self.stream))
 
     def adam(self, p, g, m, v, n, lr, step, b1=0.9, b2=0.999, eps=1e-8):
@@ -283,15 +289,29 @@ class FNO:
         b (partials), then reduced by one product with a ones vector."""
         o = self.ops
         K = self.N * self.N
-        kc = min(K, 1024)
+        # short chunks: m, n <= width, so the partial products need many
+        # workgroups (K / kc per case) to fill the chip
+        kc = min(K, 128)
         S = K // kc
         part = self._buf(B * S, m, n)
         for b in range(B):
             o.mm(A, Bm, part, m, n, kc, (a_rc[0], a_rc[1], kc * a_rc[1]), (b_rc[0], b_rc[1], kc * b_rc[0]),
                  (n, 1, m * n), batch=S, offs=(b * a_bstride, b * b_bstride, b * S * m * n))
-        self._ensure_ones(max(K, B * S))
-        o.mm(part, self.ones, self.g[gname], m * n, 1, B * S, (1, m * n, 0), (1, 1, 0), (1, 1, 0), beta=1.0,
-             offs=(0, 0, off))
+        self._reduce_rows(part, B * S, m * n, self.g[gname], off)
+
+    def _reduce_rows(self, part, rows, cols, out, off, group=64):
+        """out[off:off+cols] += sum over the rows of part (rows x cols, row-major).
+        Long sums are taken in two stages (groups of `group` rows batched over all
+        groups, then the group sums), so the product never runs thousands of
+        k-tiles on a handful of workgroups; the order of the sums is fixed."""
+        o = self.ops
+        self._ensure_ones(max(rows, group))
+        if rows > 4 * group and rows % group == 0:
+            ng = rows // group
+            mid = self._buf(ng, cols)
+            o.mm(part, self.ones, mid, cols, 1, group, (1, cols, group * cols), (1, 1, 0), (1, 1, cols), batch=ng)
+            part, rows = mid, ng
+        o.mm(part, self.ones, out, cols, 1, rows, (1, cols, 0), (1, 1, 0), (1, 1, 0), beta=1.0, offs=(0, 0, off))
 
     def _bias_grad(self, d, gname, C, B, NN, off=0):
         """g[gname][off:off+C] += sum over batch and pixels of d (B, C, NN)."""
