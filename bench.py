@@ -368,7 +368,7 @@ def warmstart_run(g, torch, args):
     ops = ws.net.ops
     ws.predict(Xte[:1])
     torch.cuda.synchronize()
-    f0 = ops.flops
+    f0, b0 = ops.flops, ops.bytes
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     reps = 20
     e0.record()
@@ -378,9 +378,10 @@ def warmstart_run(g, torch, args):
     torch.cuda.synchronize()
     inf_ms = e0.elapsed_time(e1) / reps
     flops = (ops.flops - f0) / reps
-    rec["fno_inference"] = dict(ms=inf_ms, flops=flops, achieved_TFs=flops / (inf_ms * 1e-3) / 1e12,
-                                peak_TFs=BF16_DENSE_PEAK_TFS,
-                                frac=flops / (inf_ms * 1e-3) / 1e12 / BF16_DENSE_PEAK_TFS)
+    nbytes = (ops.bytes - b0) / reps
+    rec["fno_inference"] = dict(ms=inf_ms, flops=flops, operand_bytes=nbytes,
+                                achieved_TFs=flops / (inf_ms * 1e-3) / 1e12,
+                                achieved_GBs=nbytes / (inf_ms * 1e-3) / 1e9)
     print(f"[warmstart] inference {rec['fno_inference']}", flush=True)
     # refinement: cold vs warm, FP64, one handle
     it = g.init(c5_problem(g, np.complex128)(p_test[0]), g.StrangSplitting(), (0.0, 0.05), dt=0.05, nsaves=1)
@@ -429,9 +430,12 @@ def warmstart_run(g, torch, args):
         "cases_other_branch": int((same > 1e-4).sum()),
         "cold_fallbacks": int(fallbacks),
         "controlled_starts": diag,
-        "roofline": {"bound": "mfma", "kernel": "gemm_kernel<bf16> (FNO inference, one case)",
-                     "achieved": rec["fno_inference"]["achieved_TFs"], "peak": BF16_DENSE_PEAK_TFS,
-                     "unit": "TFLOP/s", "frac": rec["fno_inference"]["frac"], "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "FNO inference, one case (gemm_kernel<bf16>, bias/GELU, "
+                                               "band limit)",
+                     "achieved": rec["fno_inference"]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rec["fno_inference"]["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "note": "about 90 launches for one case: launch latency, not bandwidth, sets this; "
+                             "bench.py --fno-bench gives the batched throughput"},
         **rec,
     }
     print(json.dumps(out))
