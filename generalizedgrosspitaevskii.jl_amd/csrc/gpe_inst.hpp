@@ -22,16 +22,41 @@ static hipError_t launch_x(const XPassArgs& a, int64_t nlines, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename R, int N, int M>
-static hipError_t launch_y(const YPassArgs& a, hipStream_t s) {
+template <typename R, int N, int M, int C>
+static hipError_t launch_y_cols(const YPassArgs& a, hipStream_t s) {
   constexpr int T = N / kE;
-  constexpr int C = y_cols_per_wg(N);
   const size_t lds = ylds_bytes<R, N, M, C>();
   auto k = y_pass_kernel<R, N, M, C>;
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3((unsigned)a.nblocks), dim3(M * C * T), lds, s, a);
   return hipGetLastError();
+}
+
+// Columns per workgroup of the y-pass, chosen per launch from two shapes:
+// C = y_cols_per_wg(N), which keeps enough workgroups for a single grid, and
+// 2C when that still leaves >= kWideMinBlocks workgroups (8 per CU).  Measured
+// on 1024 trajectories of 256x256 (FP32, noise): 280 us per launch at C = 4,
+// 250 us at C = 8, 352 us at C = 2; a single 512x512 grid is fastest at C = 2
+// (6.9 us wave span against 8.5 at C = 4 and 14.8 at C = 8).
+constexpr int64_t kWideMinBlocks = 2048;
+
+template <typename R, int N, int M>
+static hipError_t launch_y(const YPassArgs& a, hipStream_t s) {
+  constexpr int T = N / kE;
+  constexpr int C = y_cols_per_wg(N);
+  constexpr int C2 = 2 * C;
+  constexpr bool wide_ok = C2 <= 16 && M * C2 * T <= 1024 && ylds_bytes<R, N, M, C2>() <= 160 * 1024;
+  if constexpr (wide_ok) {
+    if (a.nx % C2 == 0) {
+      YPassArgs b = a;
+      b.ncolblk = a.nx / C2;
+      b.nblocks = b.ncolblk * (a.nblocks / a.ncolblk);
+      b.xcd_remap = (b.nblocks % 8 == 0) ? 1 : 0;
+      if (b.nblocks >= kWideMinBlocks) return launch_y_cols<R, N, M, C2>(b, s);
+    }
+  }
+  return launch_y_cols<R, N, M, C>(a, s);
 }
 
 template <typename R, int N, int M>

@@ -173,6 +173,17 @@ def test_2d_batched_two_component(ggp, orc):
     check(*out, FP64_TOL)
 
 
+def test_2d_batched_wide_column_shape(ggp, orc):
+    """Enough trajectories that the column pass takes its wide workgroup shape
+    (16 columns of 128 for 16 x 128 grids: 2048 workgroups), with the 2x2
+    dispersion exchange between components."""
+    rng = np.random.default_rng(21)
+    shape = (16, 128, 2048)
+    u0 = tuple((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.1 for _ in range(2))
+    out = run_both(ggp, orc, u0, (4.0, 32.0), (0, 0.1), 0.05, 1, **polariton_terms(32.0))
+    check(*out, FP64_TOL)
+
+
 # -------------------------------------------------------------- noise (TWA)
 @pytest.mark.parametrize("noise_dtype", [np.complex128, np.float64])
 def test_noise_1d_philox(ggp, orc, noise_dtype):
