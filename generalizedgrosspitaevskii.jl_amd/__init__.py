@@ -8,11 +8,12 @@ user API.  The directory name is not a Python identifier, so load it with
 """
 from .problem import (GrossPitaevskiiProblem, PhiloxRNG, SMatrix, StrangSplitting, StrangSplittingIterator,
                       SVector, device_empty, direct_grid, init, pump_times, reciprocal_grid,
-                      resolve_fixed_timestepping, solve, solve_sweep, steady_state, SweepIterator)
+                      resolve_fixed_timestepping, solve, solve_sweep, steady_state, SweepIterator,
+                      wigner_density_g2)
 from ._lib import GpeError, GpeUnsupported, load as load_library
 
 __all__ = [
     "GrossPitaevskiiProblem", "StrangSplitting", "StrangSplittingIterator", "solve", "init", "SVector", "SMatrix",
-    "PhiloxRNG", "solve_sweep", "steady_state", "SweepIterator", "direct_grid", "reciprocal_grid", "resolve_fixed_timestepping", "pump_times", "device_empty",
+    "PhiloxRNG", "solve_sweep", "steady_state", "SweepIterator", "wigner_density_g2", "direct_grid", "reciprocal_grid", "resolve_fixed_timestepping", "pump_times", "device_empty",
     "GpeError", "GpeUnsupported", "load_library",
 ]

@@ -645,6 +645,28 @@ class StrangSplittingIterator:
                                       ms=ms.value)
 
 
+def wigner_density_g2(n, n2, commutator):
+    """Physical density and equal-point g2 from truncated-Wigner moments.
+
+    n, n2: ensemble means of |psi|^2 and |psi|^4 per point (as returned by
+    solve(..., ensemble_moments=True)); commutator: [psi, psi^dagger] of the
+    discretised field (1/dA on a grid of cell area dA).  Symmetric ordering
+    is undone as in examples/truncated_wigner.jl:110-165 (there for psi(k)
+    pairs with delta_{k,k'}), here for the same point (delta = 1):
+        n_phys = <|psi|^2> - c/2
+        G2     = <|psi|^4> - 2 c <|psi|^2> + c^2/2
+        g2     = G2 / n_phys^2
+    Returns (n_phys, g2)."""
+    n = np.asarray(n, dtype=np.float64)
+    n2 = np.asarray(n2, dtype=np.float64)
+    c = float(commutator)
+    n_phys = n - c / 2
+    G2 = n2 - 2 * c * n + c * c / 2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g2 = G2 / (n_phys * n_phys)
+    return n_phys, g2
+
+
 def steady_state(prob, alg, *, dt, tol=1e-8, check_every=50, max_steps=200000, u0=None):
     """Run prob (static pump) from u0 (default prob.u0) until the relative
     step-to-step change ||u_{n+1} - u_n|| / ||u_{n+1}|| <= tol.  No reference

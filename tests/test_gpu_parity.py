@@ -301,3 +301,6 @@ def test_twa_stationary_vacuum_occupation(ggp, dtype):
         assert abs(n1[0][..., f].mean() / target - 1) < 0.02
         # complex Gaussian field: <|psi|^4> = 2 <|psi|^2>^2
         assert abs(n2[0][..., f].mean() / (2 * n1[0][..., f].mean() ** 2) - 1) < 0.03
+        # symmetric ordering undone (examples/truncated_wigner.jl:110-165): no particles, no pairs
+        n_phys, _ = ggp.wigner_density_g2(n1[0][..., f].mean(), n2[0][..., f].mean(), 1 / dA)
+        assert abs(n_phys) < 0.02 * target

@@ -89,3 +89,28 @@ def test_noise_prototype_without_noise_func_is_an_error(ggp):
         ggp.init(prob, ggp.StrangSplitting(), (0, 1), dt=0.1, nsaves=1)
     with pytest.raises(TypeError):
         ggp.init(prob, object(), (0, 1), dt=0.1, nsaves=1)
+
+
+def test_wigner_density_g2_closed_forms(ggp):
+    """Symmetric-ordering correction of the truncated-Wigner moments on
+    synthetic Wigner samples with known answers: vacuum (n = 0, no pairs),
+    coherent state (g2 = 1), thermal state (g2 = 2)."""
+    rng = np.random.default_rng(11)
+    c = 1 / 0.0625  # commutator 1/dA
+    S = 400000
+
+    def cgauss(var):
+        return np.sqrt(var / 2) * (rng.standard_normal(S) + 1j * rng.standard_normal(S))
+
+    def moments(psi):
+        a = np.abs(psi) ** 2
+        return a.mean(), (a * a).mean()
+
+    n, g2 = ggp.wigner_density_g2(*moments(cgauss(c / 2)), c)
+    assert abs(n) < 0.01 * c
+    alpha = 3.0 * np.sqrt(c)
+    n, g2 = ggp.wigner_density_g2(*moments(alpha + cgauss(c / 2)), c)
+    assert abs(n / alpha ** 2 - 1) < 0.01 and abs(g2 - 1) < 0.01
+    nbar = 2.0 * c
+    n, g2 = ggp.wigner_density_g2(*moments(cgauss(nbar + c / 2)), c)
+    assert abs(n / nbar - 1) < 0.02 and abs(g2 - 2) < 0.05
