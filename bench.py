@@ -441,6 +441,48 @@ def warmstart_run(g, torch, args):
     print(json.dumps(out))
 
 
+def size_sweep(g, torch, sizes=(256, 512, 1024, 2048, 4096), target_s=0.5):
+    """The headline workload (two-component polariton, FP32 fields, device
+    frames every 100 steps) on N x N grids with the grid spacing kept
+    (L = 100 N / 512): where the path leaves the latency regime.  Each size
+    runs as many steps (a multiple of 100) as fill about target_s."""
+    rows = []
+    for n in sizes:
+        L = L_BOX * n / N_GRID
+        shape = (n, n)
+        u0 = []
+        for _ in range(2):
+            t = g.device_empty(shape, "complex64", 0)
+            t.zero_()
+            u0.append(t)
+        p = polariton_param()
+        p.update(L=L)
+        prob = g.GrossPitaevskiiProblem(tuple(u0), (L, L), dispersion=dispersion, nonlinearity=nonlinearity,
+                                        pump=pump, param=p)
+        it = g.init(prob, g.StrangSplitting(), (0.0, 100 * 0.05), dt=0.05, nsaves=1)
+        it.solve(save_frames=False)  # warm-up and per-step estimate
+        torch.cuda.synchronize()
+        est = max(it.last_ms() / 100, 1e-3)
+        it.close()
+        steps = max(100, int(target_s * 1e3 / est) // 100 * 100)
+        it = g.init(prob, g.StrangSplitting(), (0.0, steps * 0.05), dt=0.05, nsaves=steps // 100)
+        res = tuple(g.device_empty(shape + (it.nsaves + 1,), "complex64", 0) for _ in range(2))
+        it.solve(result=res)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it.solve(result=res)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        it.close()
+        del res
+        gps = n * n * steps / wall
+        rows.append(dict(n=n, steps=steps, us_per_step=wall * 1e6 / steps, gps=gps,
+                         hbm_equiv_frac=gps * 64 / 1e9 / HBM_PEAK_GBS, state_mib=2 * n * n * 8 / 2 ** 20))
+        print(f"  {n}x{n}: {rows[-1]['us_per_step']:.1f} us/step, {gps:.3e} gps/s, "
+              f"{100 * rows[-1]['hbm_equiv_frac']:.1f} % of HBM at 64 B/gps", flush=True)
+    return {"metric": "grid-point-steps/s by grid size (two-component polariton, FP32)", "rows": rows}
+
+
 def fno_bench(torch, batch=16, reps=10):
     """Throughput of the warm-start network's kernels alone: the 256x256 FNO
     (width 32, 16 modes, 4 layers, random weights) forward on `batch` cases
@@ -554,6 +596,7 @@ def main():
     ap.add_argument("--ws-load", default=None, help="skip data and training, load a surrogate (.npz)")
     ap.add_argument("--ws-diag", type=int, default=0, help="controlled-start comparison on this many test cases")
     ap.add_argument("--fno-bench", action="store_true", help="run only the warm-start network's throughput")
+    ap.add_argument("--size-sweep", action="store_true", help="run only the headline workload at 256..4096 squared")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (PMC calibration)")
     args = ap.parse_args()
@@ -580,6 +623,9 @@ def main():
         return
     if args.fno_bench:
         print(json.dumps(fno_bench(torch)))
+        return
+    if args.size_sweep:
+        print(json.dumps(size_sweep(g, torch)))
         return
     runs = {}
     for dtype in ([args.dtype] + [d for d in ("complex64", "complex128") if d != args.dtype]):
