@@ -35,6 +35,10 @@ GPE_NOISE_NONE = 0
 GPE_NOISE_COMPLEX = 1
 GPE_NOISE_REAL = 2
 
+GPE_SCHEDULE_AUTO = 0
+GPE_SCHEDULE_LAUNCHES = 1
+GPE_SCHEDULE_GRAPH = 2
+
 
 class ProblemDesc(ctypes.Structure):
     _fields_ = [
@@ -80,6 +84,8 @@ SIGNATURES = {
     "gpe_pump_times": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, P, I64]),
     "gpe_solve": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, I32,
                                   ctypes.POINTER(P), P]),
+    "gpe_set_schedule": (ctypes.c_int, [P, I32]),
+    "gpe_get_schedule": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
     "gpe_get_field": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "gpe_last_solve_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
     "gpe_advance": (ctypes.c_int, [P, I64, ctypes.POINTER(ctypes.c_float)]),

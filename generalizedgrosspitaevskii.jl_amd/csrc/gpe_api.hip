@@ -108,6 +108,9 @@ struct gpe_handle {
   double event_pair_ms = 0;  // latency of an empty HIP event pair (gpe_profile_kernels)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   SteadyBufs steady;
+  // launch schedule of 2D solves (GPE_SCHEDULE_*) and the one last used
+  int schedule = GPE_SCHEDULE_AUTO;
+  int last_schedule = GPE_SCHEDULE_LAUNCHES;
 };
 
 static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -687,6 +690,49 @@ extern "C" int gpe_set_observables(gpe_handle* h, double* n, double* n2) {
   return GPE_OK;
 }
 
+// The 2D step loop as a captured HIP graph: steps_per_graph repetitions of
+// (y-pass, x-pass) with the arguments of a step inside a save block.  Those
+// arguments are the same for every such step when no noise is drawn and the
+// envelope is static (the Philox counter in HalfStep::call is then unused),
+// so one graph, replayed once per save block, issues exactly the launches of
+// the per-step loop.  Capture always ends, also after a failed launch.
+static int capture_steps(gpe_handle* h, const XPassArgs& xa, const YPassArgs& ya, int64_t steps_per_graph,
+                         hipGraphExec_t* exec) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+  hipError_t e = hipSuccess;
+  for (int64_t s = 0; s < steps_per_graph && e == hipSuccess; ++s) {
+    e = run_y(h, ya);
+    if (e == hipSuccess) e = run_x(h, xa);
+  }
+  const hipError_t e_end = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess || e_end != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(GPE_ERR_DEVICE, "gpe_solve: step graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e_end));
+  }
+  e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(GPE_ERR_DEVICE, "gpe_solve: step graph instantiation: %s", hipGetErrorString(e));
+  return GPE_OK;
+}
+
+// Whether a 2D solve replays a step graph.  The capture costs about as much
+// host time as issuing the launches of the captured steps once, and each
+// replay saves ~0.3-0.9 us per step on the device (measured on MI355X from
+// 64x64 to 512x512), so AUTO replays when there are at least kGraphMinSaves
+// save blocks.
+constexpr int64_t kGraphMinSaves = 8;
+
+static bool graph_eligible(const gpe_handle* h, int64_t sps) {
+  return h->ndim == 2 && h->noise_kind == GPE_NOISE_NONE && h->env.empty() && sps >= 2;
+}
+
+static bool use_step_graph(const gpe_handle* h, int64_t sps, int64_t nsaves) {
+  if (!graph_eligible(h, sps) || h->schedule == GPE_SCHEDULE_LAUNCHES) return false;
+  if (h->schedule == GPE_SCHEDULE_GRAPH) return true;
+  return nsaves >= kGraphMinSaves;
+}
+
 extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double dt, int64_t nsaves,
                          int32_t save_start, void* const* result, double* ts) {
   if (!h || !u0) return fail(GPE_ERR_INVALID, "gpe_solve: null argument");
@@ -715,6 +761,10 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     }
     HIPCHK(hipMemcpyAsync(h->d_env, h->env.data(), need * sizeof(double), hipMemcpyHostToDevice, h->stream));
   }
+  const bool graph = use_step_graph(h, sps, nsaves);
+  if (h->schedule == GPE_SCHEDULE_GRAPH && !graph)
+    return fail(GPE_ERR_UNSUPPORTED,
+                "gpe_solve: the graph schedule needs a 2D grid, no noise, a static envelope and >= 2 steps per save");
   rc = prepare_tables(h, rdt);
   if (rc) return rc;
   const size_t fb = field_bytes(h);
@@ -784,8 +834,51 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
   const uint64_t call0 = h->call;
   double t = t0;
   if (ts) ts[0] = t0;
+  hipGraphExec_t gexec = nullptr;
+  if (graph) {
+    XPassArgs xg = x_args(h);
+    xg.flags = kInK | kEnd | kStart | kOutK;
+    xg.h_end = make_half(h, rdt, 1, 1, call0);
+    xg.h_start = make_half(h, rdt, 2, 0, call0);
+    rc = capture_steps(h, xg, y_args(h), sps - 1, &gexec);
+    if (rc) return rc;
+  }
+  // the graph is released on every return path below
+  struct ExecGuard {
+    hipGraphExec_t& e;
+    ~ExecGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+    }
+  } exec_guard{gexec};
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  if (h->ndim == 2) {
+  if (graph) {
+    // per save block: the replayed steps, then the block's last step with its
+    // frame (and, at the end of the solve, without the start half-step)
+    XPassArgs xa = x_args(h);
+    const YPassArgs ya = y_args(h);
+    xa.flags = kStart | kOutK;
+    xa.h_start = make_half(h, rdt, 1, 0, call0);
+    HIPCHK(run_x(h, xa));
+    for (int64_t n = 0; n < nsaves; ++n) {
+      HIPCHK(hipGraphLaunch(gexec, h->stream));
+      const int64_t s = (n + 1) * sps;
+      for (int64_t k = 0; k < sps; ++k) t += rdt;
+      HIPCHK(run_y(h, ya));
+      const int64_t f = n + (save_start ? 1 : 0);
+      xa.h_end = make_half(h, rdt, s, 1, call0);
+      for (int c = 0; c < kMaxM; ++c) xa.frame[c] = c < h->M ? frame_ptr(c, f) : nullptr;
+      if (s < S) {
+        xa.flags = kInK | kEnd | kStart | kOutK;
+        xa.h_start = make_half(h, rdt, s + 1, 0, call0);
+      } else {
+        xa.flags = kInK | kEnd;
+      }
+      HIPCHK(run_x(h, xa));
+      rc = flush_frame(f);
+      if (rc) return rc;
+      if (ts) ts[n + (save_start ? 1 : 0)] = t;
+    }
+  } else if (h->ndim == 2) {
     XPassArgs xa = x_args(h);
     const YPassArgs ya = y_args(h);
     xa.flags = kStart | kOutK;
@@ -837,6 +930,21 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
   HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
   h->configured = true;
   h->dt = rdt;
+  if (h->ndim == 2) h->last_schedule = graph ? GPE_SCHEDULE_GRAPH : GPE_SCHEDULE_LAUNCHES;
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_schedule(gpe_handle* h, int32_t schedule) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_schedule: null handle");
+  if (schedule != GPE_SCHEDULE_AUTO && schedule != GPE_SCHEDULE_LAUNCHES && schedule != GPE_SCHEDULE_GRAPH)
+    return fail(GPE_ERR_INVALID, "gpe_set_schedule: schedule=%d", schedule);
+  h->schedule = schedule;
+  return GPE_OK;
+}
+
+extern "C" int gpe_get_schedule(gpe_handle* h, int32_t* schedule) {
+  if (!h || !schedule) return fail(GPE_ERR_INVALID, "gpe_get_schedule: null argument");
+  *schedule = h->last_schedule;
   return GPE_OK;
 }
 

@@ -498,6 +498,25 @@ class StrangSplittingIterator:
         self.prob = new
         self._set_terms(terms)
 
+    SCHEDULES = {"auto": L.GPE_SCHEDULE_AUTO, "launches": L.GPE_SCHEDULE_LAUNCHES, "graph": L.GPE_SCHEDULE_GRAPH}
+
+    def set_schedule(self, schedule):
+        """How solve issues the 2D step loop: "launches" (two kernel launches
+        per step), "graph" (the steps of a save block captured once as a HIP
+        graph and replayed per block; no noise, static envelope) or "auto"
+        (the graph from 8 save blocks on).  The results are identical bit for
+        bit.  No reference counterpart (the reference steps on the host)."""
+        if schedule not in self.SCHEDULES:
+            raise ValueError(f"schedule must be one of {sorted(self.SCHEDULES)}")
+        L.check(self._lib.gpe_set_schedule(self.h, self.SCHEDULES[schedule]))
+
+    @property
+    def last_schedule(self):
+        """The schedule the last 2D solve ran with ("launches" or "graph")."""
+        v = ctypes.c_int32()
+        L.check(self._lib.gpe_get_schedule(self.h, ctypes.byref(v)))
+        return {code: name for name, code in self.SCHEDULES.items()}[v.value]
+
     def _set_terms(self, which=TERMS):
         prob, lib, h = self.prob, self._lib, self.h
         M, spatial = prob.M, self.spatial

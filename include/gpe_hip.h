@@ -141,6 +141,24 @@ int gpe_pump_times(double t0, double t1, double dt, int64_t nsaves, double* time
 int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double dt,
               int64_t nsaves, int32_t save_start, void* const* result, double* ts);
 
+/* How gpe_solve issues the 2D step loop.  The reference steps on the host,
+ * one broadcast kernel per operation (src/fixed_time_stepping.jl:40-46,
+ * src/strang_splitting.jl:86-90); the choice has no counterpart there.
+ *   GPE_SCHEDULE_LAUNCHES: two launches per step (column pass, row pass).
+ *   GPE_SCHEDULE_GRAPH:    the steps inside each save block are captured once
+ *                          per solve as a HIP graph and replayed per block.
+ *                          Needs no noise, a static envelope and >= 2 steps
+ *                          per save (GPE_ERR_UNSUPPORTED otherwise).
+ *   GPE_SCHEDULE_AUTO:     the graph when eligible and nsaves >= 8 (default).
+ * The kernels and their arguments are the same under every schedule, so the
+ * results are identical bit for bit. */
+enum gpe_schedule { GPE_SCHEDULE_AUTO = 0, GPE_SCHEDULE_LAUNCHES = 1, GPE_SCHEDULE_GRAPH = 2 };
+int gpe_set_schedule(gpe_handle* h, int32_t schedule);
+
+/* The schedule the last 2D gpe_solve ran with (GPE_SCHEDULE_LAUNCHES or
+ * GPE_SCHEDULE_GRAPH). */
+int gpe_get_schedule(gpe_handle* h, int32_t* schedule);
+
 /* Copy the field after the last solve (real space) into u[c]. */
 int gpe_get_field(gpe_handle* h, void* const* u);
 

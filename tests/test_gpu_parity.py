@@ -304,3 +304,85 @@ def test_twa_stationary_vacuum_occupation(ggp, dtype):
         # symmetric ordering undone (examples/truncated_wigner.jl:110-165): no particles, no pairs
         n_phys, _ = ggp.wigner_density_g2(n1[0][..., f].mean(), n2[0][..., f].mean(), 1 / dA)
         assert abs(n_phys) < 0.02 * target
+
+
+# ------------------------------------------------------------ step graph
+def _solve_with(ggp, prob, tspan, dt, nsaves, schedule, save_start=True, moments=False):
+    it = ggp.init(prob, ggp.StrangSplitting(), tspan, dt=dt, nsaves=nsaves, save_start=save_start)
+    try:
+        it.set_schedule(schedule)
+        out = it.solve(ensemble_moments=moments)
+        return out, it.last_schedule
+    finally:
+        it.close()
+
+
+def _graph_case(name):
+    if name == "kerr64_f64":
+        u0, lengths = c1_inputs(64)
+        terms = dict(dispersion=disp_kin, nonlinearity=lambda u, p: -6.0 * abs(u[0]) ** 2)
+        return u0, lengths, terms, np.complex128, (0, 0.5), 0.01, 10, False
+    if name == "polariton128_f32":
+        L = 100.0 * 128 / 512
+        u0 = (np.zeros((128, 128), complex), np.zeros((128, 128), complex))
+        return u0, (L, L), polariton_terms(L), np.complex64, (0, 5.0), 0.05, 10, False
+    # batched 32x32 x 4, potential, ensemble moments of every frame
+    N, B, L = 32, 4, 8.0
+    rs = np.arange(N) * (L / N)
+    base = np.exp(-(rs[:, None] - 4) ** 2 - (rs[None, :] - 3) ** 2).astype(complex)
+    u0 = (np.stack([base * (1 + 0.1 * b) for b in range(B)], axis=2),)
+    terms = dict(dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2 - 0.3j,
+                 potential=lambda r, p: 0.5 * ((r[0] - 4) ** 2 + (r[1] - 4) ** 2),
+                 nonlinearity=lambda u, p: abs(u[0]) ** 2)
+    return u0, (L, L), terms, np.complex128, (0, 1.0), 0.02, 10, True
+
+
+@pytest.mark.parametrize("save_start", [True, False])
+@pytest.mark.parametrize("case", ["kerr64_f64", "polariton128_f32", "batched32_f64_potential_moments"])
+def test_step_graph_matches_launches_bit_for_bit(ggp, orc, case, save_start):
+    """The replayed step graph issues the per-step launches with the same
+    arguments, so frames, ts and ensemble moments agree bit for bit with the
+    launch-per-step schedule (every frame is one replay), and the FP64 frames
+    meet the oracle bar."""
+    u0, lengths, terms, dtype, tspan, dt, nsaves, moments = _graph_case(case)
+    prob = ggp.GrossPitaevskiiProblem(tuple(np.asfortranarray(x.astype(dtype)) for x in u0), lengths, **terms)
+    a, sched_a = _solve_with(ggp, prob, tspan, dt, nsaves, "launches", save_start, moments)
+    b, sched_b = _solve_with(ggp, prob, tspan, dt, nsaves, "graph", save_start, moments)
+    assert (sched_a, sched_b) == ("launches", "graph")
+    assert np.array_equal(a[0], b[0])
+    for x, y in zip(a[1], b[1]):
+        assert np.array_equal(x, y)
+    if moments:
+        for pa, pb in zip(a[2], b[2]):
+            for x, y in zip(pa, pb):
+                assert np.array_equal(x, y)
+    if dtype == np.complex128:
+        ts_o, res_o, _ = orc.solve(orc.Problem(u0, lengths, **terms), tspan, dt, nsaves, save_start)
+        check(ts_o, res_o, b[0], b[1], FP64_TOL)
+
+
+def test_step_graph_schedule_rules(ggp):
+    """AUTO replays a graph from 8 save blocks of >= 2 steps on; an explicit
+    graph schedule refuses noise (each step draws new Philox counters)."""
+    u0, lengths = c1_inputs(32)
+    prob = ggp.GrossPitaevskiiProblem(u0, lengths, dispersion=disp_kin,
+                                      nonlinearity=lambda u, p: -1.0 * abs(u[0]) ** 2)
+    for nsaves, want in ((8, "graph"), (4, "launches")):
+        it = ggp.init(prob, ggp.StrangSplitting(), (0, 0.16), dt=0.01, nsaves=nsaves)
+        it.solve()
+        assert it.last_schedule == want, (nsaves, it.last_schedule)
+        it.close()
+    it = ggp.init(prob, ggp.StrangSplitting(), (0, 0.08), dt=0.01, nsaves=8)  # one step per save
+    it.solve()
+    assert it.last_schedule == "launches"
+    it.close()
+    noisy = ggp.GrossPitaevskiiProblem(u0, lengths, dispersion=disp_kin, position_noise_func=lambda u, r, p: 0.1,
+                                       noise_prototype=(np.empty((32, 32), complex),))
+    it = ggp.init(noisy, ggp.StrangSplitting(), (0, 0.16), dt=0.01, nsaves=8, rng=ggp.PhiloxRNG(1))
+    it.set_schedule("graph")
+    with pytest.raises(ggp.GpeUnsupported):
+        it.solve()
+    it.set_schedule("auto")
+    it.solve()
+    assert it.last_schedule == "launches"
+    it.close()
