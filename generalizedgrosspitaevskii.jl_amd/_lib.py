@@ -39,6 +39,9 @@ GPE_SCHEDULE_AUTO = 0
 GPE_SCHEDULE_LAUNCHES = 1
 GPE_SCHEDULE_GRAPH = 2
 
+GPE_PLAN_ESTIMATE = 0
+GPE_PLAN_MEASURE = 1
+
 
 class ProblemDesc(ctypes.Structure):
     _fields_ = [
@@ -86,6 +89,9 @@ SIGNATURES = {
                                   ctypes.POINTER(P), P]),
     "gpe_set_schedule": (ctypes.c_int, [P, I32]),
     "gpe_get_schedule": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
+    "gpe_set_planner": (ctypes.c_int, [P, I32]),
+    "gpe_set_plan": (ctypes.c_int, [P, I32, I32]),
+    "gpe_get_plan": (ctypes.c_int, [P, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32), D]),
     "gpe_get_field": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "gpe_last_solve_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
     "gpe_advance": (ctypes.c_int, [P, I64, ctypes.POINTER(ctypes.c_float)]),

@@ -111,6 +111,15 @@ struct gpe_handle {
   // launch schedule of 2D solves (GPE_SCHEDULE_*) and the one last used
   int schedule = GPE_SCHEDULE_AUTO;
   int last_schedule = GPE_SCHEDULE_LAUNCHES;
+  // launch shapes of the 2D passes (rows / columns per workgroup; 0: the
+  // built-in rule), how they were chosen, and the timings of the last
+  // measurement (us per launch, indexed like plan_cand_*)
+  int planner = GPE_PLAN_MEASURE;
+  int plan_w = 0, plan_c = 0;
+  bool plan_fixed = false;
+  uint64_t plan_key = 0;  // terms the measured plan was timed with (0: none)
+  int plan_cand_w[2] = {0, 0}, plan_cand_c[3] = {0, 0, 0};
+  double plan_us_w[2] = {0, 0}, plan_us_c[3] = {0, 0, 0};
 };
 
 static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -127,13 +136,16 @@ static bool is_device_ptr(const void* p) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
-static hipError_t run_x(gpe_handle* h, const XPassArgs& a) {
+static hipError_t run_x_w(gpe_handle* h, const XPassArgs& a, int w) {
   const int64_t nlines = h->npts / h->n0;
-  return h->f64 ? gpe_run_x_f64(h->M, h->n0, a, nlines, h->stream) : gpe_run_x_f32(h->M, h->n0, a, nlines, h->stream);
+  return h->f64 ? gpe_run_x_f64(h->M, h->n0, a, nlines, w, h->stream)
+                : gpe_run_x_f32(h->M, h->n0, a, nlines, w, h->stream);
 }
-static hipError_t run_y(gpe_handle* h, const YPassArgs& a) {
-  return h->f64 ? gpe_run_y_f64(h->M, h->n1, a, h->stream) : gpe_run_y_f32(h->M, h->n1, a, h->stream);
+static hipError_t run_y_c(gpe_handle* h, const YPassArgs& a, int c) {
+  return h->f64 ? gpe_run_y_f64(h->M, h->n1, a, c, h->stream) : gpe_run_y_f32(h->M, h->n1, a, c, h->stream);
 }
+static hipError_t run_x(gpe_handle* h, const XPassArgs& a) { return run_x_w(h, a, h->plan_w); }
+static hipError_t run_y(gpe_handle* h, const YPassArgs& a) { return run_y_c(h, a, h->plan_c); }
 static hipError_t run_1d(gpe_handle* h, const Step1DArgs& a) {
   return h->f64 ? gpe_run_1d_f64(h->M, h->n0, a, h->stream) : gpe_run_1d_f32(h->M, h->n0, a, h->stream);
 }
@@ -668,6 +680,145 @@ static Step1DArgs s1_args(gpe_handle* h, double dt) {
   return a;
 }
 
+// ---------------------------------------------------------------- planner
+// The measured counterpart of FFTW's planner (the reference plans with
+// FFTW.jl's default, src/misc.jl:53-58): the kernels' arithmetic is fixed, so
+// the plan is the launch shape of the two 2D passes.  GPE_PLAN_MEASURE times
+// every valid shape on the handle itself, with its tables and terms, before
+// the first solve (and again when the terms that select kernel code change),
+// and keeps the fastest.  The shapes change no arithmetic: results are
+// identical bit for bit under every plan.
+static uint64_t plan_key_of(const gpe_handle* h) {
+  return 1 + (uint64_t)(h->noise_kind != GPE_NOISE_NONE) + 2 * (uint64_t)h->g_coupled + 4 * (uint64_t)h->g_cplx +
+         8 * (uint64_t)h->gmask + 32 * (uint64_t)h->disp_form + 256 * (uint64_t)h->pot_form +
+         4096 * (uint64_t)h->pump_kind;
+}
+
+// Mean device time of one launch of fn, from HIP events on the handle's
+// stream: two warm-up launches, a three-launch estimate, then enough launches
+// for about 2 ms (3..200).
+static int time_launch(gpe_handle* h, const std::function<hipError_t()>& fn, double* us) {
+  for (int i = 0; i < 2; ++i) HIPCHK(fn());
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < 3; ++i) HIPCHK(fn());
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const double est = std::max((double)ms / 3, 1e-4);
+  const int reps = (int)std::min(200.0, std::max(3.0, 2.0 / est));
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < reps; ++i) HIPCHK(fn());
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *us = 1e3 * ms / reps;
+  return GPE_OK;
+}
+
+// Called with the tables prepared and before the start field is copied in:
+// the candidates run in place on a zeroed field.
+static int ensure_plan(gpe_handle* h, double dt) {
+  if (h->ndim != 2 || h->planner != GPE_PLAN_MEASURE || h->plan_fixed) return GPE_OK;
+  const uint64_t key = plan_key_of(h);
+  if (h->plan_key == key) return GPE_OK;
+  for (int c = 0; c < h->M; ++c) HIPCHK(hipMemsetAsync(h->u[c], 0, field_bytes(h), h->stream));
+  XPassArgs xa = x_args(h);
+  xa.flags = kInK | kEnd | kStart | kOutK;
+  xa.h_end = make_half(h, dt, 1, 1, h->call);
+  xa.h_start = make_half(h, dt, 2, 0, h->call);
+  const YPassArgs ya = y_args(h);
+  const int n0 = (int)h->n0, n1 = (int)h->n1;
+  const int W = x_lines_per_wg(n0), C = y_cols_per_wg(n1);
+  const int cw[2] = {W, 2 * W};
+  const int cc[3] = {C > 1 ? C / 2 : 0, C, 2 * C};
+  int best_w = W, best_c = y_rule_cols(n1, h->M, h->f64, h->n0, h->batch);
+  double tw = INFINITY, tc = INFINITY;
+  for (int i = 0; i < 2; ++i) {
+    h->plan_cand_w[i] = 0;
+    h->plan_us_w[i] = 0;
+    if (!x_plan_valid(n0, h->M, h->f64, cw[i])) continue;
+    double us = 0;
+    const int w = cw[i];
+    int rc = time_launch(h, [&] { return run_x_w(h, xa, w); }, &us);
+    if (rc) return rc;
+    h->plan_cand_w[i] = w;
+    h->plan_us_w[i] = us;
+    if (us < tw) {
+      tw = us;
+      best_w = w;
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    h->plan_cand_c[i] = 0;
+    h->plan_us_c[i] = 0;
+    if (cc[i] < 1 || !y_plan_valid(n1, h->M, h->f64, cc[i]) || h->n0 % cc[i] != 0) continue;
+    double us = 0;
+    const int c = cc[i];
+    int rc = time_launch(h, [&] { return run_y_c(h, ya, c); }, &us);
+    if (rc) return rc;
+    h->plan_cand_c[i] = c;
+    h->plan_us_c[i] = us;
+    if (us < tc) {
+      tc = us;
+      best_c = c;
+    }
+  }
+  h->plan_w = best_w;
+  h->plan_c = best_c;
+  h->plan_key = key;
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_planner(gpe_handle* h, int32_t planner) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_planner: null handle");
+  if (planner != GPE_PLAN_ESTIMATE && planner != GPE_PLAN_MEASURE)
+    return fail(GPE_ERR_INVALID, "gpe_set_planner: planner=%d", planner);
+  h->planner = planner;
+  h->plan_fixed = false;
+  h->plan_key = 0;
+  h->plan_w = h->plan_c = 0;
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_plan(gpe_handle* h, int32_t x_rows, int32_t y_cols) {
+  if (!h) return fail(GPE_ERR_INVALID, "gpe_set_plan: null handle");
+  if (h->ndim != 2 && (x_rows != 0 || y_cols != 0))
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_set_plan: launch shapes are chosen for 2D grids only");
+  if (x_rows != 0 && !x_plan_valid((int)h->n0, h->M, h->f64, x_rows))
+    return fail(GPE_ERR_INVALID, "gpe_set_plan: %d rows per workgroup is not a valid x-pass shape for n0=%lld",
+                x_rows, (long long)h->n0);
+  if (y_cols != 0 && (!y_plan_valid((int)h->n1, h->M, h->f64, y_cols) || h->n0 % y_cols != 0))
+    return fail(GPE_ERR_INVALID, "gpe_set_plan: %d columns per workgroup is not a valid y-pass shape for n1=%lld",
+                y_cols, (long long)h->n1);
+  h->plan_w = x_rows;
+  h->plan_c = y_cols;
+  h->plan_fixed = true;
+  return GPE_OK;
+}
+
+extern "C" int gpe_get_plan(gpe_handle* h, int32_t* x_rows, int32_t* y_cols, int32_t* cand, double* us) {
+  if (!h || !x_rows || !y_cols) return fail(GPE_ERR_INVALID, "gpe_get_plan: null argument");
+  if (h->ndim != 2) {
+    *x_rows = x_lines_per_wg((int)h->n0);
+    *y_cols = 0;
+  } else {
+    *x_rows = h->plan_w ? h->plan_w : x_lines_per_wg((int)h->n0);
+    *y_cols = h->plan_c ? h->plan_c : y_rule_cols((int)h->n1, h->M, h->f64, h->n0, h->batch);
+  }
+  if (cand && us) {
+    for (int i = 0; i < 2; ++i) {
+      cand[i] = h->plan_cand_w[i];
+      us[i] = h->plan_us_w[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+      cand[2 + i] = h->plan_cand_c[i];
+      us[2 + i] = h->plan_us_c[i];
+    }
+  }
+  return GPE_OK;
+}
+
 static int ensure_stage(gpe_handle* h) {
   for (int c = 0; c < h->M; ++c)
     if (!h->stage[c]) HIPCHK(hipMalloc(&h->stage[c], field_bytes(h)));
@@ -766,6 +917,8 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     return fail(GPE_ERR_UNSUPPORTED,
                 "gpe_solve: the graph schedule needs a 2D grid, no noise, a static envelope and >= 2 steps per save");
   rc = prepare_tables(h, rdt);
+  if (rc) return rc;
+  rc = ensure_plan(h, rdt);
   if (rc) return rc;
   const size_t fb = field_bytes(h);
   for (int c = 0; c < h->M; ++c) HIPCHK(hipMemcpyAsync(h->u[c], u0[c], fb, hipMemcpyDefault, h->stream));
@@ -1430,6 +1583,8 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
   if (!h->env.empty()) return fail(GPE_ERR_STATE, "gpe_steady_state: the pump envelope must be static");
   HIPCHK(hipSetDevice(h->d.device));
   int rc = prepare_tables(h, dt);
+  if (rc) return rc;
+  rc = ensure_plan(h, dt);
   if (rc) return rc;
   const size_t fb = field_bytes(h);
   SteadyBufs& sb = h->steady;

@@ -10,9 +10,8 @@ static hipError_t set_lds(K k, size_t lds) {
   return hipSuccess;
 }
 
-template <typename R, int N, int M>
-static hipError_t launch_x(const XPassArgs& a, int64_t nlines, hipStream_t s) {
-  constexpr int W = x_lines_per_wg(N);
+template <typename R, int N, int M, int W>
+static hipError_t launch_x_rows(const XPassArgs& a, int64_t nlines, hipStream_t s) {
   constexpr int T = N / kE;
   const size_t lds = xlds_bytes<R, N, M, W>();
   auto k = a.pt.noise_kind ? x_pass_kernel<R, N, M, W, true> : x_pass_kernel<R, N, M, W, false>;
@@ -20,6 +19,18 @@ static hipError_t launch_x(const XPassArgs& a, int64_t nlines, hipStream_t s) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3((unsigned)((nlines + W - 1) / W)), dim3(M * W * T), lds, s, a);
   return hipGetLastError();
+}
+
+// Rows per workgroup of the x-pass: w == 0 or x_lines_per_wg(N) is the
+// built-in shape; a measured plan may also take twice as many
+// (x_plan_valid).  Other values are rejected.
+template <typename R, int N, int M>
+static hipError_t launch_x(const XPassArgs& a, int64_t nlines, int w, hipStream_t s) {
+  constexpr int W = x_lines_per_wg(N);
+  if (w == 0 || w == W) return launch_x_rows<R, N, M, W>(a, nlines, s);
+  if constexpr (x_plan_valid(N, M, sizeof(R) == 8, 2 * W))
+    if (w == 2 * W) return launch_x_rows<R, N, M, 2 * W>(a, nlines, s);
+  return hipErrorInvalidValue;
 }
 
 template <typename R, int N, int M, int C>
@@ -33,30 +44,33 @@ static hipError_t launch_y_cols(const YPassArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Columns per workgroup of the y-pass, chosen per launch from two shapes:
-// C = y_cols_per_wg(N), which keeps enough workgroups for a single grid, and
-// 2C when that still leaves >= kWideMinBlocks workgroups (8 per CU).  Measured
-// on 1024 trajectories of 256x256 (FP32, noise): 280 us per launch at C = 4,
-// 250 us at C = 8, 352 us at C = 2; a single 512x512 grid is fastest at C = 2
-// (6.9 us wave span against 8.5 at C = 4 and 14.8 at C = 8).
-constexpr int64_t kWideMinBlocks = 2048;
+// The y-pass with C' columns per workgroup; `a` carries the block counts of
+// y_cols_per_wg(N) columns and is re-derived for C'.
+template <typename R, int N, int M, int CP>
+static hipError_t launch_y_recount(const YPassArgs& a, hipStream_t s) {
+  if (a.nx % CP != 0) return hipErrorInvalidValue;
+  YPassArgs b = a;
+  b.ncolblk = a.nx / CP;
+  b.nblocks = b.ncolblk * (a.nblocks / a.ncolblk);
+  b.xcd_remap = (b.nblocks % 8 == 0) ? 1 : 0;
+  return launch_y_cols<R, N, M, CP>(b, s);
+}
 
+// c == 0: the built-in rule (y_rule_cols); otherwise exactly c columns, one
+// of C/2, C, 2C where y_plan_valid allows it.
 template <typename R, int N, int M>
-static hipError_t launch_y(const YPassArgs& a, hipStream_t s) {
-  constexpr int T = N / kE;
+static hipError_t launch_y(const YPassArgs& a, int c, hipStream_t s) {
   constexpr int C = y_cols_per_wg(N);
   constexpr int C2 = 2 * C;
-  constexpr bool wide_ok = C2 <= 16 && M * C2 * T <= 1024 && ylds_bytes<R, N, M, C2>() <= 160 * 1024;
-  if constexpr (wide_ok) {
-    if (a.nx % C2 == 0) {
-      YPassArgs b = a;
-      b.ncolblk = a.nx / C2;
-      b.nblocks = b.ncolblk * (a.nblocks / a.ncolblk);
-      b.xcd_remap = (b.nblocks % 8 == 0) ? 1 : 0;
-      if (b.nblocks >= kWideMinBlocks) return launch_y_cols<R, N, M, C2>(b, s);
-    }
-  }
-  return launch_y_cols<R, N, M, C>(a, s);
+  constexpr int CH = C > 1 ? C / 2 : 1;
+  constexpr bool wide_ok = y_plan_valid(N, M, sizeof(R) == 8, C2);
+  if (c == 0) c = y_rule_cols(N, M, sizeof(R) == 8, a.nx, a.nblocks / a.ncolblk);
+  if (c == C) return launch_y_cols<R, N, M, C>(a, s);
+  if constexpr (wide_ok)
+    if (c == C2) return launch_y_recount<R, N, M, C2>(a, s);
+  if constexpr (C > 1 && y_plan_valid(N, M, sizeof(R) == 8, CH))
+    if (c == CH) return launch_y_recount<R, N, M, CH>(a, s);
+  return hipErrorInvalidValue;
 }
 
 template <typename R, int N, int M>
@@ -88,13 +102,13 @@ static hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
 #define GPE_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
 #define GPE_DEFINE_RUNNERS(R, SUFFIX)                                                                  \
-  hipError_t gpe_run_x_##SUFFIX(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, hipStream_t s) { \
+  hipError_t gpe_run_x_##SUFFIX(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s) { \
     switch (N * 4 + M) {                                                                               \
       GPE_SIZES(GPE_XCASE_##SUFFIX)                                                                    \
     }                                                                                                  \
     return hipErrorInvalidValue;                                                                       \
   }                                                                                                    \
-  hipError_t gpe_run_y_##SUFFIX(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s) {           \
+  hipError_t gpe_run_y_##SUFFIX(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s) {    \
     switch (N * 4 + M) {                                                                               \
       GPE_SIZES(GPE_YCASE_##SUFFIX)                                                                    \
     }                                                                                                  \

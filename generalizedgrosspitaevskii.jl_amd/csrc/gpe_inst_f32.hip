@@ -2,11 +2,11 @@
 #include "gpe_inst.hpp"
 
 #define GPE_XCASE_f32(n)                                       \
-  case n * 4 + 1: return gpe::launch_x<float, n, 1>(a, nlines, s); \
-  case n * 4 + 2: return gpe::launch_x<float, n, 2>(a, nlines, s);
+  case n * 4 + 1: return gpe::launch_x<float, n, 1>(a, nlines, w, s); \
+  case n * 4 + 2: return gpe::launch_x<float, n, 2>(a, nlines, w, s);
 #define GPE_YCASE_f32(n)                                  \
-  case n * 4 + 1: return gpe::launch_y<float, n, 1>(a, s); \
-  case n * 4 + 2: return gpe::launch_y<float, n, 2>(a, s);
+  case n * 4 + 1: return gpe::launch_y<float, n, 1>(a, c, s); \
+  case n * 4 + 2: return gpe::launch_y<float, n, 2>(a, c, s);
 #define GPE_1CASE_f32(n)                                   \
   case n * 4 + 1: return gpe::launch_1d<float, n, 1>(a, s); \
   case n * 4 + 2: return gpe::launch_1d<float, n, 2>(a, s);

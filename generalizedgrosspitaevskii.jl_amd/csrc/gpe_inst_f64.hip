@@ -3,11 +3,11 @@
 #include "gpe_inst.hpp"
 
 #define GPE_XCASE_f64(n)                                        \
-  case n * 4 + 1: return gpe::launch_x<double, n, 1>(a, nlines, s); \
-  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_x<double, (n > 2048 ? 2048 : n), 2>(a, nlines, s);
+  case n * 4 + 1: return gpe::launch_x<double, n, 1>(a, nlines, w, s); \
+  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_x<double, (n > 2048 ? 2048 : n), 2>(a, nlines, w, s);
 #define GPE_YCASE_f64(n)                                   \
-  case n * 4 + 1: return gpe::launch_y<double, n, 1>(a, s); \
-  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_y<double, (n > 2048 ? 2048 : n), 2>(a, s);
+  case n * 4 + 1: return gpe::launch_y<double, n, 1>(a, c, s); \
+  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_y<double, (n > 2048 ? 2048 : n), 2>(a, c, s);
 #define GPE_1CASE_f64(n)                                    \
   case n * 4 + 1: return gpe::launch_1d<double, n, 1>(a, s); \
   case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_1d<double, (n > 2048 ? 2048 : n), 2>(a, s);

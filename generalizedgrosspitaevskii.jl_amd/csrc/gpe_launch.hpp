@@ -15,12 +15,50 @@ __host__ __device__ constexpr int y_cols_per_wg(int N) {
   return (128 / (N / kE)) < 1 ? 1 : ((128 / (N / kE)) > 16 ? 16 : 128 / (N / kE));
 }
 
+// Bytes of the two LDS buffers of `lines` padded lines of N complex values.
+__host__ __device__ constexpr size_t pass_lds_bytes(int N, int lines, bool f64) {
+  return 2ull * lines * lds_line_stride(N) * (f64 ? 16 : 8);
+}
+
+// Shapes a plan may give the x-pass: x_lines_per_wg(N) rows per workgroup or
+// twice as many, within 1024 threads and 160 KiB of LDS.
+__host__ __device__ constexpr bool x_plan_valid(int N, int M, bool f64, int w) {
+  return (w == x_lines_per_wg(N) || w == 2 * x_lines_per_wg(N)) && M * w * (N / kE) <= 1024 &&
+         pass_lds_bytes(N, M * w, f64) <= 160 * 1024;
+}
+
+// Shapes a plan may give the y-pass: C = y_cols_per_wg(N) columns per
+// workgroup, 2C, or C/2 while that still fills a wavefront per component; at
+// most 16 columns, 1024 threads and 160 KiB of LDS.
+__host__ __device__ constexpr bool y_plan_valid(int N, int M, bool f64, int c) {
+  return (c == y_cols_per_wg(N) || c == 2 * y_cols_per_wg(N) ||
+          (y_cols_per_wg(N) > 1 && c == y_cols_per_wg(N) / 2 && c * (N / kE) >= 64)) &&
+         c >= 1 && c <= 16 && M * c * (N / kE) <= 1024 && pass_lds_bytes(N, M * c, f64) <= 160 * 1024;
+}
+
+// The built-in column count of a y-pass over nx columns of `batch` grids:
+// C = y_cols_per_wg(N), which keeps enough workgroups for a single grid, and
+// 2C when that still leaves >= kWideMinBlocks workgroups (8 per CU).
+// Measured on 1024 trajectories of 256x256 (FP32, noise): 280 us per launch
+// at C = 4, 250 us at C = 8, 352 us at C = 2; a single 512x512 grid is
+// fastest at C = 2 (6.9 us wave span against 8.5 at C = 4 and 14.8 at C = 8).
+constexpr int64_t kWideMinBlocks = 2048;
+__host__ __device__ constexpr int y_rule_cols(int N, int M, bool f64, int64_t nx, int64_t batch) {
+  return (y_plan_valid(N, M, f64, 2 * y_cols_per_wg(N)) && nx % (2 * y_cols_per_wg(N)) == 0 &&
+          (nx / (2 * y_cols_per_wg(N))) * batch >= kWideMinBlocks)
+             ? 2 * y_cols_per_wg(N)
+             : y_cols_per_wg(N);
+}
+
 }  // namespace gpe
 
-hipError_t gpe_run_x_f32(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, hipStream_t s);
-hipError_t gpe_run_x_f64(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, hipStream_t s);
-hipError_t gpe_run_y_f32(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s);
-hipError_t gpe_run_y_f64(int M, int64_t N, const gpe::YPassArgs& a, hipStream_t s);
+// Pass launchers: w / c rows / columns per workgroup, 0 for the built-in
+// shape (x_lines_per_wg, y_rule_cols); other values must satisfy
+// x_plan_valid / y_plan_valid (hipErrorInvalidValue otherwise).
+hipError_t gpe_run_x_f32(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s);
+hipError_t gpe_run_x_f64(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s);
+hipError_t gpe_run_y_f32(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s);
+hipError_t gpe_run_y_f64(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s);
 hipError_t gpe_run_sweep_f32(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
 hipError_t gpe_run_sweep_f64(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
 hipError_t gpe_run_1d_f32(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);

@@ -510,6 +510,36 @@ class StrangSplittingIterator:
             raise ValueError(f"schedule must be one of {sorted(self.SCHEDULES)}")
         L.check(self._lib.gpe_set_schedule(self.h, self.SCHEDULES[schedule]))
 
+    PLANNERS = {"estimate": L.GPE_PLAN_ESTIMATE, "measure": L.GPE_PLAN_MEASURE}
+
+    def set_planner(self, planner):
+        """Launch-shape planner of the 2D passes, the counterpart of the FFTW
+        planner flags behind get_fft_plans (src/misc.jl:53-58): "estimate"
+        (built-in shapes) or "measure" (the default: the next solve first
+        times every valid shape on this handle and keeps the fastest).
+        Results are identical under every plan."""
+        if planner not in self.PLANNERS:
+            raise ValueError(f"planner must be one of {sorted(self.PLANNERS)}")
+        L.check(self._lib.gpe_set_planner(self.h, self.PLANNERS[planner]))
+
+    def set_plan(self, x_rows=0, y_cols=0):
+        """Fix the rows per workgroup of the row pass and the columns per
+        workgroup of the column pass (0: built-in), e.g. a measured plan."""
+        L.check(self._lib.gpe_set_plan(self.h, int(x_rows), int(y_cols)))
+
+    @property
+    def plan(self):
+        """dict(x_rows, y_cols) in use, and the last measurement's mean us per
+        launch of every candidate shape ({} before any measurement)."""
+        xr, yc = ctypes.c_int32(), ctypes.c_int32()
+        cand = (ctypes.c_int32 * 5)()
+        us = np.zeros(5)
+        L.check(self._lib.gpe_get_plan(self.h, ctypes.byref(xr), ctypes.byref(yc), cand,
+                                       us.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        x = {int(cand[i]): float(us[i]) for i in range(2) if cand[i]}
+        y = {int(cand[i]): float(us[i]) for i in range(2, 5) if cand[i]}
+        return dict(x_rows=xr.value, y_cols=yc.value, measured_us=dict(x_rows=x, y_cols=y) if x or y else {})
+
     @property
     def last_schedule(self):
         """The schedule the last 2D solve ran with ("launches" or "graph")."""

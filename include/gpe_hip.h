@@ -159,6 +159,34 @@ int gpe_set_schedule(gpe_handle* h, int32_t schedule);
  * GPE_SCHEDULE_GRAPH). */
 int gpe_get_schedule(gpe_handle* h, int32_t* schedule);
 
+/* Planner of the 2D passes' launch shapes, the counterpart of the FFTW
+ * planner behind get_fft_plans (src/misc.jl:53-58, FFTW.jl's default flags):
+ *   GPE_PLAN_ESTIMATE: the built-in shapes.
+ *   GPE_PLAN_MEASURE:  (default) the next gpe_solve / gpe_steady_state of a
+ *                      2D handle first times every
+ *                      valid shape of the row pass (rows per workgroup) and
+ *                      of the column pass (columns per workgroup) on this
+ *                      handle, with its tables and terms, and keeps the
+ *                      fastest; it measures again when the terms that select
+ *                      kernel code (noise, nonlinearity, dispersion,
+ *                      potential or pump kind) change.
+ * Setting the planner drops any plan fixed by gpe_set_plan.  No shape changes
+ * the arithmetic: results are identical bit for bit under every plan. */
+enum gpe_planner { GPE_PLAN_ESTIMATE = 0, GPE_PLAN_MEASURE = 1 };
+int gpe_set_planner(gpe_handle* h, int32_t planner);
+
+/* Fix the launch shapes (e.g. a plan measured before, the counterpart of
+ * FFTW wisdom): x_rows rows per workgroup of the row pass and y_cols columns
+ * per workgroup of the column pass, 0 for the built-in shape.  2D only.
+ * GPE_ERR_INVALID for a shape the kernels do not provide. */
+int gpe_set_plan(gpe_handle* h, int32_t x_rows, int32_t y_cols);
+
+/* The shapes in use.  cand[5] and us[5] (both may be NULL) receive the
+ * candidates of the last measurement and their mean time per launch in us:
+ * entries 0-1 row-pass shapes, 2-4 column-pass shapes (0 = not a valid
+ * candidate, or not measured). */
+int gpe_get_plan(gpe_handle* h, int32_t* x_rows, int32_t* y_cols, int32_t* cand, double* us);
+
 /* Copy the field after the last solve (real space) into u[c]. */
 int gpe_get_field(gpe_handle* h, void* const* u);
 

@@ -22,7 +22,8 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
-def run_both(ggp, orc, u0, lengths, tspan, dt, nsaves, dtype=np.complex128, save_start=True, seed=None, **terms):
+def run_both(ggp, orc, u0, lengths, tspan, dt, nsaves, dtype=np.complex128, save_start=True, seed=None, planner=None,
+             **terms):
     ts_o, res_o, _ = orc.solve(orc.Problem(u0, lengths, noise_seed=seed or 0, **_oracle_terms(terms)), tspan, dt,
                                nsaves, save_start)
     gu0 = tuple(np.asfortranarray(x.astype(dtype)) for x in u0)
@@ -32,8 +33,12 @@ def run_both(ggp, orc, u0, lengths, tspan, dt, nsaves, dtype=np.complex128, save
     else:
         gterms.pop("noise_dtype", None)
     rng = ggp.PhiloxRNG(seed or 0) if seed is not None else None
-    ts, res = ggp.solve(ggp.GrossPitaevskiiProblem(gu0, lengths, **gterms), ggp.StrangSplitting(), tspan, dt=dt,
-                        nsaves=nsaves, save_start=save_start, rng=rng)
+    it = ggp.init(ggp.GrossPitaevskiiProblem(gu0, lengths, **gterms), ggp.StrangSplitting(), tspan, dt=dt,
+                  nsaves=nsaves, save_start=save_start, rng=rng)
+    if planner is not None:
+        it.set_planner(planner)
+    ts, res = it.solve()
+    it.close()
     return ts_o, res_o, ts, res
 
 
@@ -180,7 +185,7 @@ def test_2d_batched_wide_column_shape(ggp, orc):
     rng = np.random.default_rng(21)
     shape = (16, 128, 2048)
     u0 = tuple((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.1 for _ in range(2))
-    out = run_both(ggp, orc, u0, (4.0, 32.0), (0, 0.1), 0.05, 1, **polariton_terms(32.0))
+    out = run_both(ggp, orc, u0, (4.0, 32.0), (0, 0.1), 0.05, 1, planner="estimate", **polariton_terms(32.0))
     check(*out, FP64_TOL)
 
 
@@ -385,4 +390,40 @@ def test_step_graph_schedule_rules(ggp):
     it.set_schedule("auto")
     it.solve()
     assert it.last_schedule == "launches"
+    it.close()
+
+
+# ------------------------------------------------------------ measured plans
+@pytest.mark.parametrize("case", ["kerr64_f64", "polariton128_f32", "batched32_f64_potential_moments"])
+def test_measured_plan_matches_estimate_bit_for_bit(ggp, case):
+    """GPE_PLAN_MEASURE times every valid launch shape on the handle and keeps
+    the fastest; the shapes change no arithmetic, so every plan, measured or
+    imposed, gives the built-in plan's frames bit for bit."""
+    u0, lengths, terms, dtype, tspan, dt, nsaves, _ = _graph_case(case)
+    prob = ggp.GrossPitaevskiiProblem(tuple(np.asfortranarray(x.astype(dtype)) for x in u0), lengths, **terms)
+    it = ggp.init(prob, ggp.StrangSplitting(), tspan, dt=dt, nsaves=nsaves)
+    it.set_planner("estimate")
+    ref = it.solve()
+    est = it.plan
+    assert est["measured_us"] == {}
+    it.set_planner("measure")
+    out = it.solve()
+    plan = it.plan
+    meas = plan["measured_us"]
+    assert plan["x_rows"] in meas["x_rows"] and plan["y_cols"] in meas["y_cols"]
+    assert plan["x_rows"] == min(meas["x_rows"], key=meas["x_rows"].get)
+    assert plan["y_cols"] == min(meas["y_cols"], key=meas["y_cols"].get)
+    assert all(v > 0 for d in meas.values() for v in d.values())
+    for x, y in zip(ref[1], out[1]):
+        assert np.array_equal(x, y)
+    shapes = [(w, c) for w in meas["x_rows"] for c in meas["y_cols"]]
+    assert len(shapes) >= 2
+    for w, c in shapes:
+        it.set_plan(w, c)
+        got = it.solve()
+        assert it.plan["x_rows"] == w and it.plan["y_cols"] == c
+        for x, y in zip(ref[1], got[1]):
+            assert np.array_equal(x, y), (w, c)
+    with pytest.raises(ggp.GpeError):
+        it.set_plan(3, 0)
     it.close()
