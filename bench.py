@@ -473,19 +473,23 @@ def size_sweep(g, torch, sizes=(256, 512, 1024, 2048, 4096), target_s=0.5):
         steps = max(100, int(target_s * 1e3 / est) // 100 * 100)
         it = g.init(prob, g.StrangSplitting(), (0.0, steps * 0.05), dt=0.05, nsaves=steps // 100)
         res = tuple(g.device_empty(shape + (it.nsaves + 1,), "complex64", 0) for _ in range(2))
-        it.solve(result=res)
+        it.solve(result=res)  # untimed: first touch of the frames; measures the launch plan
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         it.solve(result=res)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        plan, sched = it.plan, it.last_schedule
         it.close()
         del res
         gps = n * n * steps / wall
         rows.append(dict(n=n, steps=steps, us_per_step=wall * 1e6 / steps, gps=gps,
-                         hbm_equiv_frac=gps * 64 / 1e9 / HBM_PEAK_GBS, state_mib=2 * n * n * 8 / 2 ** 20))
+                         hbm_equiv_frac=gps * 64 / 1e9 / HBM_PEAK_GBS, state_mib=2 * n * n * 8 / 2 ** 20,
+                         plan=plan, step_loop=sched))
         print(f"  {n}x{n}: {rows[-1]['us_per_step']:.1f} us/step, {gps:.3e} gps/s, "
-              f"{100 * rows[-1]['hbm_equiv_frac']:.1f} % of HBM at 64 B/gps", flush=True)
+              f"{100 * rows[-1]['hbm_equiv_frac']:.1f} % of HBM at 64 B/gps; plan (x_rows, y_cols) = "
+              f"({plan['x_rows']}, {plan['y_cols']}), measured us per launch {plan['measured_us']}, {sched}",
+              flush=True)
     return {"metric": "grid-point-steps/s by grid size (two-component polariton, FP32)", "rows": rows}
 
 
