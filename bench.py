@@ -215,7 +215,7 @@ def sweep_run(g, torch, ninst=4096, steps=4000, dtype=np.complex64):
     it.close()
     pts = int(sizes.sum())
     return dict(workload=f"uneven-size bistability sweep, {ninst} x 1D N in {{256..2048}}, {steps} steps, "
-                         "one launch per size and 20-step block", value=pts * steps / wall,
+                         "one launch per length and 20-step block", value=pts * steps / wall,
                 unit="grid-point-steps/s", ms_total=wall * 1e3, device_ms=dev_ms, steps=steps, points=pts,
                 dtype="c64" if dtype == np.complex64 else "c128", host_setup_s=t_setup,
                 bound="VALU (each line stays in registers for 20 steps; HBM traffic 16 B per point per 20 steps)")

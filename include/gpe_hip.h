@@ -150,13 +150,15 @@ int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double
  *                          Needs no noise, a static envelope and >= 2 steps
  *                          per save (GPE_ERR_UNSUPPORTED otherwise).
  *   GPE_SCHEDULE_AUTO:     the graph when eligible and nsaves >= 8 (default).
- * The kernels and their arguments are the same under every schedule, so the
- * results are identical bit for bit. */
+ * gpe_steady_state replays the steps between residual checks the same way
+ * (no noise, check_every >= 4) unless the schedule is LAUNCHES.  The kernels
+ * and their arguments are the same under every schedule, so the results are
+ * identical bit for bit. */
 enum gpe_schedule { GPE_SCHEDULE_AUTO = 0, GPE_SCHEDULE_LAUNCHES = 1, GPE_SCHEDULE_GRAPH = 2 };
 int gpe_set_schedule(gpe_handle* h, int32_t schedule);
 
-/* The schedule the last 2D gpe_solve ran with (GPE_SCHEDULE_LAUNCHES or
- * GPE_SCHEDULE_GRAPH). */
+/* The schedule the last 2D gpe_solve or gpe_steady_state ran with
+ * (GPE_SCHEDULE_LAUNCHES or GPE_SCHEDULE_GRAPH). */
 int gpe_get_schedule(gpe_handle* h, int32_t* schedule);
 
 /* Planner of the 2D passes' launch shapes, the counterpart of the FFTW

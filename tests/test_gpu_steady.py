@@ -51,3 +51,25 @@ def test_steady_state_converges(ggp):
                         ggp.StrangSplitting(), (0, 5.0), dt=0.05, nsaves=1)
     for c in range(2):
         assert np.linalg.norm(res[c][..., -1] - u[c]) <= 2e-4 * np.linalg.norm(u[c])
+
+
+def test_steady_state_graph_replay_is_bit_identical(ggp):
+    """Without noise the steps of a check block before the two compared
+    fields replay as one captured graph; residual history, step count and
+    final field equal the launch-per-step run bit for bit."""
+    N, L = 64, 25.0
+    u0 = (np.zeros((N, N), complex), np.zeros((N, N), complex))
+    prob = ggp.GrossPitaevskiiProblem(u0, (L, L), **polariton_terms(L, A=1.0, w=6.0))
+    out = {}
+    for sched in ("launches", "auto"):
+        it = ggp.init(prob, ggp.StrangSplitting(), (0.0, 0.05), dt=0.05, nsaves=1)
+        it.set_schedule(sched)
+        u, info = it.steady_state(tol=1e-300, check_every=25, max_steps=200)
+        out[sched] = (u, info, it.last_schedule)
+        it.close()
+    (ua, ia, sa), (ub, ib, sb) = out["launches"], out["auto"]
+    assert (sa, sb) == ("launches", "graph")
+    assert ia["steps"] == ib["steps"] == 200
+    assert np.array_equal(ia["history"], ib["history"])
+    for c in range(2):
+        assert np.array_equal(ua[c], ub[c])
