@@ -81,8 +81,10 @@ def bracket_overhead(raw_x, raw_y, step_ms):
     return o, o
 
 
-def gpu_run(g, torch, dtype, steps, warmup, dt=0.05):
-    """Returns dict with wall/device timing and per-kernel durations."""
+def gpu_run(g, torch, dtype, steps, warmup, dt=0.05, plan=None):
+    """Returns dict with wall/device timing and per-kernel durations.  plan =
+    (x_rows, y_cols) fixes the launch shapes; None lets the first handle
+    measure them and reuses its choice."""
     shape = (N_GRID, N_GRID)
     u0 = []
     for _ in range(2):
@@ -96,7 +98,9 @@ def gpu_run(g, torch, dtype, steps, warmup, dt=0.05):
         return g.init(prob, g.StrangSplitting(), (0.0, nsteps * dt), dt=dt, nsaves=nsaves)
 
     it_w = make_iter(warmup)
-    it_w.solve()  # also measures the launch plan
+    if plan is not None:
+        it_w.set_plan(*plan)
+    it_w.solve()  # measures the launch plan unless one is given
     plan = it_w.plan
     it_w.close()
     it = make_iter(steps)
@@ -485,10 +489,14 @@ def size_sweep(g, torch, sizes=(256, 512, 1024, 2048, 4096), target_s=0.5):
         gps = n * n * steps / wall
         rows.append(dict(n=n, steps=steps, us_per_step=wall * 1e6 / steps, gps=gps,
                          hbm_equiv_frac=gps * 64 / 1e9 / HBM_PEAK_GBS, state_mib=2 * n * n * 8 / 2 ** 20,
-                         plan=plan, step_loop=sched))
+                         plan=dict(x_rows=plan["x_rows"], y_cols=plan["y_cols"],
+                                   measured_us_per_step={f"{k[0]},{k[1]}": v
+                                                         for k, v in plan["measured_us"].items()}),
+                         step_loop=sched))
         print(f"  {n}x{n}: {rows[-1]['us_per_step']:.1f} us/step, {gps:.3e} gps/s, "
               f"{100 * rows[-1]['hbm_equiv_frac']:.1f} % of HBM at 64 B/gps; plan (x_rows, y_cols) = "
-              f"({plan['x_rows']}, {plan['y_cols']}), measured us per launch {plan['measured_us']}, {sched}",
+              f"({plan['x_rows']}, {plan['y_cols']}), measured us per step "
+              f"{ {k: round(v, 2) for k, v in plan['measured_us'].items()} }, {sched}",
               flush=True)
     return {"metric": "grid-point-steps/s by grid size (two-component polariton, FP32)", "rows": rows}
 
@@ -593,6 +601,9 @@ def main():
     ap.add_argument("--no-twa", action="store_true", help="skip the batched truncated-Wigner line")
     ap.add_argument("--twa-steps", type=int, default=2000)
     ap.add_argument("--no-sweep", action="store_true", help="skip the uneven-size sweep line")
+    ap.add_argument("--plan", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
+                    help="x_rows,y_cols: fix the headline's launch shapes instead of measuring them (profiling "
+                         "runs use it: a profiler's per-dispatch cost skews the planner's timings)")
     ap.add_argument("--dtype", default="complex64", choices=["complex64", "complex128"],
                     help="field precision of the headline run")
     ap.add_argument("--warmstart", action="store_true",
@@ -639,7 +650,7 @@ def main():
         return
     runs = {}
     for dtype in ([args.dtype] + [d for d in ("complex64", "complex128") if d != args.dtype]):
-        runs[dtype] = gpu_run(g, torch, dtype, args.steps, args.warmup)
+        runs[dtype] = gpu_run(g, torch, dtype, args.steps, args.warmup, plan=args.plan)
     head = runs[args.dtype]
     npts = N_GRID * N_GRID
     gps = npts * args.steps / head["wall"]
@@ -666,7 +677,8 @@ def main():
                                "frame every 100 steps", "grid": [N_GRID, N_GRID], "components": 2,
                    "global_batch": 1, "parallelism": "single GPU",
                    "launch_plan": {"x_rows": head["plan"]["x_rows"], "y_cols": head["plan"]["y_cols"],
-                                   "planner": "measure"}, "step_loop": head["schedule"]},
+                                   "planner": "measure" if args.plan is None else "fixed"},
+                   "step_loop": head["schedule"]},
         "hbm_equiv": {"bytes_per_gps": b_alg_gps, "achieved_GBs": gps * b_alg_gps / 1e9,
                       "frac_of_peak": gps * b_alg_gps / 1e9 / HBM_PEAK_GBS},
         "device_ms_per_step": head["dev_ms"] / args.steps,

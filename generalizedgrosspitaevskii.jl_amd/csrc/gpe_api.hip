@@ -51,6 +51,8 @@ struct SteadyBufs {
   double* host = nullptr;  // pinned pair (diff^2, norm^2)
 };
 
+constexpr int kPlanMaxCand = 6;  // (rows, columns) pairs a measured plan times
+
 struct gpe_handle {
   gpe_problem_desc d;
   int M = 1, ndim = 1;
@@ -112,14 +114,15 @@ struct gpe_handle {
   int schedule = GPE_SCHEDULE_AUTO;
   int last_schedule = GPE_SCHEDULE_LAUNCHES;
   // launch shapes of the 2D passes (rows / columns per workgroup; 0: the
-  // built-in rule), how they were chosen, and the timings of the last
-  // measurement (us per launch, indexed like plan_cand_*)
+  // built-in rule), how they were chosen, and the last measurement: the
+  // (rows, columns) pairs timed and their us per step
   int planner = GPE_PLAN_MEASURE;
   int plan_w = 0, plan_c = 0;
   bool plan_fixed = false;
   uint64_t plan_key = 0;  // terms the measured plan was timed with (0: none)
-  int plan_cand_w[2] = {0, 0}, plan_cand_c[3] = {0, 0, 0};
-  double plan_us_w[2] = {0, 0}, plan_us_c[3] = {0, 0, 0};
+  int plan_ncand = 0;
+  int plan_cand[kPlanMaxCand][2] = {};
+  double plan_us[kPlanMaxCand] = {};
 };
 
 static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -694,9 +697,9 @@ static uint64_t plan_key_of(const gpe_handle* h) {
          4096 * (uint64_t)h->pump_kind;
 }
 
-// Mean device time of one launch of fn, from HIP events on the handle's
-// stream: two warm-up launches, a three-launch estimate, then enough launches
-// for about 2 ms (3..200).
+// Mean device time of one call of fn (the launches it issues), from HIP
+// events on the handle's stream: two warm-up calls, a three-call estimate,
+// then enough calls for about 2 ms (3..200).
 static int time_launch(gpe_handle* h, const std::function<hipError_t()>& fn, double* us) {
   for (int i = 0; i < 2; ++i) HIPCHK(fn());
   HIPCHK(hipEventRecord(h->ev0, h->stream));
@@ -732,38 +735,45 @@ static int ensure_plan(gpe_handle* h, double dt) {
   const int W = x_lines_per_wg(n0), C = y_cols_per_wg(n1);
   const int cw[2] = {W, 2 * W};
   const int cc[3] = {C > 1 ? C / 2 : 0, C, 2 * C};
-  int best_w = W, best_c = y_rule_cols(n1, h->M, h->f64, h->n0, h->batch);
-  double tw = INFINITY, tc = INFINITY;
+  // Whole steps (column pass, then row pass) are timed, not the passes
+  // alone: a row pass repeated without the column pass's 1/(Nx Ny) scale
+  // lets the field grow until the half-step's sine/cosine leaves its fast
+  // range, which is not the work of a real step.
+  int n = 0;
   for (int i = 0; i < 2; ++i) {
-    h->plan_cand_w[i] = 0;
-    h->plan_us_w[i] = 0;
     if (!x_plan_valid(n0, h->M, h->f64, cw[i])) continue;
-    double us = 0;
-    const int w = cw[i];
-    int rc = time_launch(h, [&] { return run_x_w(h, xa, w); }, &us);
-    if (rc) return rc;
-    h->plan_cand_w[i] = w;
-    h->plan_us_w[i] = us;
-    if (us < tw) {
-      tw = us;
-      best_w = w;
+    for (int k = 0; k < 3; ++k) {
+      if (cc[k] < 1 || !y_plan_valid(n1, h->M, h->f64, cc[k]) || h->n0 % cc[k] != 0) continue;
+      h->plan_cand[n][0] = cw[i];
+      h->plan_cand[n][1] = cc[k];
+      h->plan_us[n] = INFINITY;
+      ++n;
     }
   }
-  for (int i = 0; i < 3; ++i) {
-    h->plan_cand_c[i] = 0;
-    h->plan_us_c[i] = 0;
-    if (cc[i] < 1 || !y_plan_valid(n1, h->M, h->f64, cc[i]) || h->n0 % cc[i] != 0) continue;
-    double us = 0;
-    const int c = cc[i];
-    int rc = time_launch(h, [&] { return run_y_c(h, ya, c); }, &us);
-    if (rc) return rc;
-    h->plan_cand_c[i] = c;
-    h->plan_us_c[i] = us;
-    if (us < tc) {
-      tc = us;
-      best_c = c;
+  h->plan_ncand = n;
+  // Two rounds over all candidates, keeping each one's faster round: the
+  // first launches on a freshly allocated field run slow, which would
+  // otherwise count against whichever shape is timed first.
+  for (int round = 0; round < 2; ++round) {
+    for (int i = 0; i < n; ++i) {
+      const int w = h->plan_cand[i][0], c = h->plan_cand[i][1];
+      double us = 0;
+      int rc = time_launch(h, [&] {
+        const hipError_t e = run_y_c(h, ya, c);
+        return e != hipSuccess ? e : run_x_w(h, xa, w);
+      }, &us);
+      if (rc) return rc;
+      h->plan_us[i] = std::min(h->plan_us[i], us);
     }
   }
+  int best_w = W, best_c = y_rule_cols(n1, h->M, h->f64, h->n0, h->batch);
+  double best = INFINITY;
+  for (int i = 0; i < n; ++i)
+    if (h->plan_us[i] < best) {
+      best = h->plan_us[i];
+      best_w = h->plan_cand[i][0];
+      best_c = h->plan_cand[i][1];
+    }
   h->plan_w = best_w;
   h->plan_c = best_c;
   h->plan_key = key;
@@ -807,13 +817,11 @@ extern "C" int gpe_get_plan(gpe_handle* h, int32_t* x_rows, int32_t* y_cols, int
     *y_cols = h->plan_c ? h->plan_c : y_rule_cols((int)h->n1, h->M, h->f64, h->n0, h->batch);
   }
   if (cand && us) {
-    for (int i = 0; i < 2; ++i) {
-      cand[i] = h->plan_cand_w[i];
-      us[i] = h->plan_us_w[i];
-    }
-    for (int i = 0; i < 3; ++i) {
-      cand[2 + i] = h->plan_cand_c[i];
-      us[2 + i] = h->plan_us_c[i];
+    for (int i = 0; i < kPlanMaxCand; ++i) {
+      const bool have = i < h->plan_ncand;
+      cand[2 * i] = have ? h->plan_cand[i][0] : 0;
+      cand[2 * i + 1] = have ? h->plan_cand[i][1] : 0;
+      us[i] = have ? h->plan_us[i] : 0;
     }
   }
   return GPE_OK;

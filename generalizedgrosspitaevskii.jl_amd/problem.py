@@ -530,15 +530,14 @@ class StrangSplittingIterator:
     @property
     def plan(self):
         """dict(x_rows, y_cols) in use, and the last measurement's mean us per
-        launch of every candidate shape ({} before any measurement)."""
+        step of every (x_rows, y_cols) pair timed ({} before any)."""
         xr, yc = ctypes.c_int32(), ctypes.c_int32()
-        cand = (ctypes.c_int32 * 5)()
-        us = np.zeros(5)
+        cand = (ctypes.c_int32 * 12)()
+        us = np.zeros(6)
         L.check(self._lib.gpe_get_plan(self.h, ctypes.byref(xr), ctypes.byref(yc), cand,
                                        us.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-        x = {int(cand[i]): float(us[i]) for i in range(2) if cand[i]}
-        y = {int(cand[i]): float(us[i]) for i in range(2, 5) if cand[i]}
-        return dict(x_rows=xr.value, y_cols=yc.value, measured_us=dict(x_rows=x, y_cols=y) if x or y else {})
+        meas = {(int(cand[2 * i]), int(cand[2 * i + 1])): float(us[i]) for i in range(6) if cand[2 * i]}
+        return dict(x_rows=xr.value, y_cols=yc.value, measured_us=meas)
 
     @property
     def last_schedule(self):

@@ -165,12 +165,12 @@ int gpe_get_schedule(gpe_handle* h, int32_t* schedule);
  * planner behind get_fft_plans (src/misc.jl:53-58, FFTW.jl's default flags):
  *   GPE_PLAN_ESTIMATE: the built-in shapes.
  *   GPE_PLAN_MEASURE:  (default) the next gpe_solve / gpe_steady_state of a
- *                      2D handle first times every
- *                      valid shape of the row pass (rows per workgroup) and
- *                      of the column pass (columns per workgroup) on this
+ *                      2D handle first times whole steps for every valid
+ *                      pair of row-pass shape (rows per workgroup) and
+ *                      column-pass shape (columns per workgroup) on this
  *                      handle, with its tables and terms, and keeps the
- *                      fastest; it measures again when the terms that select
- *                      kernel code (noise, nonlinearity, dispersion,
+ *                      fastest pair; it measures again when the terms that
+ *                      select kernel code (noise, nonlinearity, dispersion,
  *                      potential or pump kind) change.
  * Setting the planner drops any plan fixed by gpe_set_plan.  No shape changes
  * the arithmetic: results are identical bit for bit under every plan. */
@@ -183,10 +183,9 @@ int gpe_set_planner(gpe_handle* h, int32_t planner);
  * GPE_ERR_INVALID for a shape the kernels do not provide. */
 int gpe_set_plan(gpe_handle* h, int32_t x_rows, int32_t y_cols);
 
-/* The shapes in use.  cand[5] and us[5] (both may be NULL) receive the
- * candidates of the last measurement and their mean time per launch in us:
- * entries 0-1 row-pass shapes, 2-4 column-pass shapes (0 = not a valid
- * candidate, or not measured). */
+/* The shapes in use.  cand[12] and us[6] (both may be NULL) receive the
+ * (x_rows, y_cols) pairs of the last measurement and their mean time per
+ * step in us (a column pass and a row pass); unused entries are 0. */
 int gpe_get_plan(gpe_handle* h, int32_t* x_rows, int32_t* y_cols, int32_t* cand, double* us);
 
 /* Copy the field after the last solve (real space) into u[c]. */

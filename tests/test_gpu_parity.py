@@ -410,13 +410,11 @@ def test_measured_plan_matches_estimate_bit_for_bit(ggp, case):
     out = it.solve()
     plan = it.plan
     meas = plan["measured_us"]
-    assert plan["x_rows"] in meas["x_rows"] and plan["y_cols"] in meas["y_cols"]
-    assert plan["x_rows"] == min(meas["x_rows"], key=meas["x_rows"].get)
-    assert plan["y_cols"] == min(meas["y_cols"], key=meas["y_cols"].get)
-    assert all(v > 0 for d in meas.values() for v in d.values())
+    assert (plan["x_rows"], plan["y_cols"]) == min(meas, key=meas.get)
+    assert all(v > 0 for v in meas.values())
     for x, y in zip(ref[1], out[1]):
         assert np.array_equal(x, y)
-    shapes = [(w, c) for w in meas["x_rows"] for c in meas["y_cols"]]
+    shapes = list(meas)
     assert len(shapes) >= 2
     for w, c in shapes:
         it.set_plan(w, c)
