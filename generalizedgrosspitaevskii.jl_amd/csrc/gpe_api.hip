@@ -123,6 +123,9 @@ struct gpe_handle {
   int plan_ncand = 0;
   int plan_cand[kPlanMaxCand][2] = {};
   double plan_us[kPlanMaxCand] = {};
+  // gpe_kspace_g2 scratch: |psi(k)|^2 per trajectory, nbar, n(k), G2
+  void* g2_buf = nullptr;
+  size_t g2_cap = 0;
 };
 
 static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -312,6 +315,7 @@ extern "C" int gpe_destroy(gpe_handle* h) {
   dfree(h->ed);
   p = h->d_env;
   dfree(p);
+  dfree(h->g2_buf);
   free_steady(h->steady);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1113,6 +1117,46 @@ extern "C" int gpe_set_schedule(gpe_handle* h, int32_t schedule) {
 extern "C" int gpe_get_schedule(gpe_handle* h, int32_t* schedule) {
   if (!h || !schedule) return fail(GPE_ERR_INVALID, "gpe_get_schedule: null argument");
   *schedule = h->last_schedule;
+  return GPE_OK;
+}
+
+extern "C" int gpe_kspace_g2(gpe_handle* h, const void* frame, double commutator, double* nk, double* g2) {
+  if (!h || !frame || !nk || !g2) return fail(GPE_ERR_INVALID, "gpe_kspace_g2: null argument");
+  if (h->ndim != 1)
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_kspace_g2: 1D ensembles only (a 2D grid's pair matrix has prod(n)^2 entries)");
+  if (!(commutator >= 0) || !std::isfinite(commutator))
+    return fail(GPE_ERR_INVALID, "gpe_kspace_g2: commutator=%g", commutator);
+  HIPCHK(hipSetDevice(h->d.device));
+  const int64_t n = h->n0, nb = h->batch;
+  const void* src = frame;
+  if (!is_device_ptr(frame)) {
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage[0], frame, field_bytes(h), hipMemcpyHostToDevice, h->stream));
+    src = h->stage[0];
+  }
+  const size_t need = (size_t)(nb * n + 2 * n + n * n) * sizeof(double);
+  int rc = ensure_buf(&h->g2_buf, &h->g2_cap, need);
+  if (rc) return rc;
+  double* a = (double*)h->g2_buf;
+  double* nbar = a + nb * n;
+  double* dnk = nbar + n;
+  double* dg2 = dnk + n;
+  KAbs2Args ka;
+  ka.u = src;
+  ka.a = a;
+  ka.batch = nb;
+  ka.tw = h->tw;
+  ka.tws = (int)(h->ntw / n);
+  HIPCHK(h->f64 ? gpe_run_kabs2_f64(n, ka, h->stream) : gpe_run_kabs2_f32(n, ka, h->stream));
+  hipLaunchKernelGGL(kspace_mean_kernel<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, (const double*)a, n,
+                     nb, commutator, nbar, dnk);
+  hipLaunchKernelGGL(kspace_g2_kernel<0>, dim3((unsigned)(n / 16), (unsigned)(n / 16)), dim3(256), 0, h->stream,
+                     (const double*)a, (const double*)nbar, n, nb, commutator, dg2);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(nk, dnk, n * sizeof(double), hipMemcpyDefault, h->stream));
+  HIPCHK(hipMemcpyAsync(g2, dg2, (size_t)n * n * sizeof(double), hipMemcpyDefault, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return GPE_OK;
 }
 

@@ -97,6 +97,18 @@ static hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename R, int N>
+static hipError_t launch_kabs2(const KAbs2Args& a, hipStream_t s) {
+  constexpr int W = x_lines_per_wg(N);
+  constexpr int T = N / kE;
+  const size_t lds = xlds_bytes<R, N, 1, W>();
+  auto k = kspace_abs2_kernel<R, N, W>;
+  hipError_t e = set_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)((a.batch + W - 1) / W)), dim3(W * T), lds, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace gpe
 
 #define GPE_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
@@ -123,6 +135,12 @@ static hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
   hipError_t gpe_run_1d_##SUFFIX(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s) {         \
     switch (N * 4 + M) {                                                                               \
       GPE_SIZES(GPE_1CASE_##SUFFIX)                                                                    \
+    }                                                                                                  \
+    return hipErrorInvalidValue;                                                                       \
+  } \
+  hipError_t gpe_run_kabs2_##SUFFIX(int64_t N, const gpe::KAbs2Args& a, hipStream_t s) {               \
+    switch (N) {                                                                                       \
+      GPE_SIZES(GPE_KCASE_##SUFFIX)                                                                    \
     }                                                                                                  \
     return hipErrorInvalidValue;                                                                       \
   }

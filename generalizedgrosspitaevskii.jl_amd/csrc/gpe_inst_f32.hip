@@ -11,6 +11,9 @@
   case n * 4 + 1: return gpe::launch_1d<float, n, 1>(a, s); \
   case n * 4 + 2: return gpe::launch_1d<float, n, 2>(a, s);
 
+#define GPE_KCASE_f32(n) \
+  case n: return gpe::launch_kabs2<float, n>(a, s);
+
 #define GPE_SCASE_f32(n) \
   case n: return gpe::launch_sweep<float, n>(a, s);
 

@@ -12,6 +12,9 @@
   case n * 4 + 1: return gpe::launch_1d<double, n, 1>(a, s); \
   case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_1d<double, (n > 2048 ? 2048 : n), 2>(a, s);
 
+#define GPE_KCASE_f64(n) \
+  case n: return gpe::launch_kabs2<double, n>(a, s);
+
 #define GPE_SCASE_f64(n) \
   case n: return gpe::launch_sweep<double, n>(a, s);
 

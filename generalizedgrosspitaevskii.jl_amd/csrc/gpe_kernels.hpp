@@ -937,6 +937,90 @@ __global__ void observables_kernel(ObsArgs a) {
 // ------------------------------------------------ steady-state residual
 // sum |a - b|^2 and sum |a|^2 over all points and components in FP64 with
 // Neumaier compensation: per-block partials, then one block folds them.
+// ---------------------------------------- momentum-space pair correlations
+// n(k) and G2(k, k') of a 1D truncated-Wigner ensemble, the observables of
+// examples/truncated_wigner.jl:110-165:
+//   ft = fft(u_b) / N per trajectory b,  a_b(k) = |ft(k)|^2,
+//   nbar(k) = mean_b a_b(k),  n(k) = nbar(k) - c/2,
+//   G2(k, k') = mean_b a_b(k) a_b(k') - (1 + delta_kk') c/2 (nbar(k) + nbar(k') - c/2)
+// with c the commutator (1/L).  Everything is in FFT order; the example's
+// fftshift is a permutation left to the caller.
+struct KAbs2Args {
+  const void* u;  // N x batch complex values (one component of one frame)
+  double* a;      // batch x N, k fastest
+  int64_t batch;
+  const void* tw;
+  int tws;
+};
+
+// a_b(k): W trajectories per workgroup, each line transformed by its T
+// threads with the fused kernels' forward Stockham FFT.
+template <typename R, int N, int W>
+__global__ void __launch_bounds__(W * (N / kE)) kspace_abs2_kernel(KAbs2Args p) {
+  constexpr int T = N / kE;
+  constexpr int LS = lds_line_stride(N);
+  constexpr bool LOCAL = T <= 64;
+  extern __shared__ unsigned char smem_raw[];
+  Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int l = tid / T, t = tid % T;
+  const int64_t line_raw = (int64_t)blockIdx.x * W + l;
+  const bool valid = line_raw < p.batch;
+  const int64_t line = valid ? line_raw : p.batch - 1;
+  const Cx<R>* uc = reinterpret_cast<const Cx<R>*>(p.u) + line * N;
+  Cx<R> v[kE];
+#pragma unroll
+  for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
+  TwReg<R, N> tr;
+  load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(p.tw), p.tws);
+  fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), lds + l * LS, 0, W * LS, t, tr);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) {
+      const double re = (double)v[m].re / N, im = (double)v[m].im / N;
+      p.a[line * N + t + m * T] = re * re + im * im;
+    }
+  }
+}
+
+// nbar(k) = mean_b a_b(k) (Neumaier-compensated) and n(k) = nbar(k) - c/2.
+// (A template so that every unit including this header may instantiate it.)
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(256) kspace_mean_kernel(const double* a, int64_t n, int64_t batch, double comm,
+                                                          double* nbar, double* nk) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s = 0, comp = 0;
+  for (int64_t b = 0; b < batch; ++b) neumaier_add(s, comp, a[b * n + k]);
+  const double m = (s + comp) / (double)batch;
+  nbar[k] = m;
+  nk[k] = m - comm / 2;
+}
+
+// G2 on 16 x 16 tiles of (k, k'): the trajectory sum of a_b(k) a_b(k') is
+// staged through LDS 16 trajectories at a time and accumulated per entry with
+// Neumaier compensation.
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(256) kspace_g2_kernel(const double* a, const double* nbar, int64_t n, int64_t batch,
+                                                        double comm, double* g2) {
+  __shared__ double am[16][17], an[16][17];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int64_t m0 = (int64_t)blockIdx.x * 16, n0 = (int64_t)blockIdx.y * 16;
+  double s = 0, comp = 0;
+  for (int64_t b0 = 0; b0 < batch; b0 += 16) {
+    const int64_t b = b0 + ty;
+    am[ty][tx] = b < batch ? a[b * n + m0 + tx] : 0.0;
+    an[ty][tx] = b < batch ? a[b * n + n0 + tx] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) neumaier_add(s, comp, am[j][tx] * an[j][ty]);
+    __syncthreads();
+  }
+  const int64_t km = m0 + tx, kn = n0 + ty;
+  const double d = km == kn ? 2.0 : 1.0;
+  g2[kn * n + km] = (s + comp) / (double)batch - d * (comm / 2) * (nbar[km] + nbar[kn] - comm / 2);
+}
+
 struct ResidualArgs {
   const void* a[kMaxM];  // u_{n+1} (real space)
   const void* b[kMaxM];  // u_n

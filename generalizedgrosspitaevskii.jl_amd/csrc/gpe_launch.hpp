@@ -63,3 +63,5 @@ hipError_t gpe_run_sweep_f32(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
 hipError_t gpe_run_sweep_f64(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
 hipError_t gpe_run_1d_f32(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);
 hipError_t gpe_run_1d_f64(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);
+hipError_t gpe_run_kabs2_f32(int64_t N, const gpe::KAbs2Args& a, hipStream_t s);
+hipError_t gpe_run_kabs2_f64(int64_t N, const gpe::KAbs2Args& a, hipStream_t s);

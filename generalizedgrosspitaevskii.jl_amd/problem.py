@@ -658,6 +658,33 @@ class StrangSplittingIterator:
         L.check(self._lib.gpe_last_solve_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
+    def kspace_correlations(self, frame, commutator):
+        """n(k) and G2(k, k') of one 1D ensemble frame, computed on the
+        device as examples/truncated_wigner.jl:110-165 does on the host:
+        ft = fft(u_b)/N per trajectory, nks = mean |ft|^2 - c/2,
+        G2[k, k'] = mean_b f(ft(k), ft(k'), k == k', c).  frame: (N, batch)
+        field of this handle's dtype (numpy or device tensor, e.g.
+        sol[0][..., f]); commutator: 1/L in the example.  Returns (nks, G2)
+        in FFT order (apply np.fft.fftshift for the example's axes)."""
+        if self.prob.N != 1:
+            raise NotImplementedError("kspace_correlations: 1D ensembles only")
+        n = self.spatial[0]
+        want = str(self.prob.u0[0].dtype).replace("torch.", "")
+        if _is_torch(frame) and frame.is_cuda:
+            f = frame if (_contig_colmajor(frame) and str(frame.dtype).replace("torch.", "") == want) \
+                else _to_colmajor(frame, want)
+            ptr = f.data_ptr()
+            _order_after_torch(f)
+        else:
+            f = np.asfortranarray(np.asarray(frame).astype(want))
+            ptr = f.ctypes.data
+        if int(np.prod(tuple(f.shape))) != n * int(np.prod(self.shape[1:] or (1,))):
+            raise ValueError(f"kspace_correlations: frame of shape {tuple(f.shape)} does not match {self.shape}")
+        nk = np.empty(n)
+        G2 = np.empty((n, n), order="F")
+        L.check(self._lib.gpe_kspace_g2(self.h, ptr, float(commutator), nk.ctypes.data, G2.ctypes.data))
+        return nk, np.ascontiguousarray(G2)
+
     def get_field(self):
         """The handle's current real-space field (host numpy arrays)."""
         out = tuple(np.empty(self.shape, dtype=str(x.dtype).replace("torch.", ""), order="F") for x in self.prob.u0)

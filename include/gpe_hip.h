@@ -188,6 +188,20 @@ int gpe_set_plan(gpe_handle* h, int32_t x_rows, int32_t y_cols);
  * step in us (a column pass and a row pass); unused entries are 0. */
 int gpe_get_plan(gpe_handle* h, int32_t* x_rows, int32_t* y_cols, int32_t* cand, double* us);
 
+/* Momentum-space occupation and pair correlation of a 1D truncated-Wigner
+ * ensemble frame, the observables of examples/truncated_wigner.jl:110-165:
+ * with ft_b = fft(u_b)/N per trajectory b (one component of one saved frame,
+ * `frame` = N x batch values in the field precision, host or device),
+ *   nk[k]        = mean_b |ft_b(k)|^2 - commutator/2
+ *   g2[k' N + k] = mean_b |ft_b(k)|^2 |ft_b(k')|^2
+ *                  - (1 + delta_kk') commutator/2 (nbar(k) + nbar(k') - commutator/2)
+ * (the example's G2k; g2(k,k') = G2 / (n(k) n(k')) is left to the caller),
+ * nbar = mean_b |ft_b|^2, all in FFT order (the example's fftshift is a
+ * permutation), FP64 with compensated trajectory sums.  commutator is 1/L
+ * in the example.  nk (N doubles) and g2 (N*N doubles) may be host or device
+ * memory.  1D handles only. */
+int gpe_kspace_g2(gpe_handle* h, const void* frame, double commutator, double* nk, double* g2);
+
 /* Copy the field after the last solve (real space) into u[c]. */
 int gpe_get_field(gpe_handle* h, void* const* u);
 

@@ -20,6 +20,7 @@ It restates, in numpy complex128 arithmetic, what the reference computes for
 * ``get_exponential``              <- src/misc.jl:12-20
 * ``evaluate_pump``                <- src/misc.jl:22-42
 * FFT over the spatial dims        <- src/misc.jl:53-64 (FFTW plan_fft / inv)
+* ``kspace_g2`` (n(k), G2(k,k'))    <- examples/truncated_wigner.jl:110-165
 
 User closures follow the reference's calling conventions, evaluated
 vectorised over the grid instead of point by point:
@@ -478,3 +479,33 @@ def solve(prob: Problem, tspan, dt, nsaves, save_start=True, callback: Optional[
         if callback is not None:
             callback(n, it)
     return (ts if save_start else ts[1:]), result, it
+
+
+# -------------------------------------------- momentum-space correlations
+def kspace_g2(frames, commutator, pairwise=True):
+    """n(k) and G2(k, k') of a 1D truncated-Wigner ensemble frame
+    (examples/truncated_wigner.jl:110-165), in FFT order (no fftshift).
+
+    frames: N x B complex array (trajectories along axis 1).  Follows the
+    example's steps: ft = fft(frames, axis 0) / N; nks = mean_b |ft|^2 - c/2;
+    G2[m, n] = mean_b f(ft[m, b], ft[n, b], m == n, c) with
+    f(x, y, d, c) = |x|^2 |y|^2 - (1 + d) c/2 (|x|^2 + |y|^2 - c/2),
+    evaluated pair by pair as the example's G2 loop does (m, n over k), or,
+    with pairwise=False, as the matrix product a a^T / B minus the same
+    corrections (for large N).  Returns (nks, G2) in FP64."""
+    u = np.asarray(frames, dtype=np.complex128)
+    N, B = u.shape
+    ft = np.fft.fft(u, axis=0) / N
+    a = np.abs(ft) ** 2
+    c = float(commutator)
+    nks = a.mean(axis=1) - c / 2
+    if not pairwise:
+        nbar = a.mean(axis=1)
+        G2 = (a @ a.T) / B - (1 + np.eye(N)) * c / 2 * (nbar[:, None] + nbar[None, :] - c / 2)
+        return nks, G2
+    G2 = np.empty((N, N))
+    for n in range(N):
+        for m in range(N):
+            d = 1.0 if m == n else 0.0
+            G2[m, n] = np.mean(a[m] * a[n] - (1 + d) * c / 2 * (a[m] + a[n] - c / 2))
+    return nks, G2

@@ -321,3 +321,45 @@ def test_reference_exciton_polariton_steady_state():
     r1 = abs(Wr - (dx + 1j * gx - g * nx) * (dc + 1j * gc) / Wr) ** 2 * nx / abs(A) ** 2
     r2 = abs(dx + 1j * gx - g * nx) ** 2 * nx / Wr ** 2 / nc
     assert abs(r1 - 1) < 3e-2 and abs(r2 - 1) < 3e-2
+
+
+# ------------------------------------------- momentum-space pair correlations
+def test_kspace_g2_coherent_plane_wave_closed_form(orc):
+    """Every trajectory the plane wave alpha exp(2 pi i k0 x / N): |ft(k)|^2 =
+    |alpha|^2 delta_{k,k0}, so n(k) and G2(k, k') have closed forms."""
+    N, B, k0, alpha, c = 16, 5, 3, 0.7 - 0.4j, 1 / 8.0
+    x = np.arange(N)
+    frames = np.repeat((alpha * np.exp(2j * np.pi * k0 * x / N))[:, None], B, axis=1)
+    nks, G2 = orc.kspace_g2(frames, c)
+    A = abs(alpha) ** 2
+    want_n = np.full(N, -c / 2)
+    want_n[k0] += A
+    assert np.allclose(nks, want_n, rtol=0, atol=1e-15)
+    want = np.full((N, N), c * c / 4)
+    np.fill_diagonal(want, c * c / 2)
+    want[k0, :] = want[:, k0] = -(c / 2) * (A - c / 2)
+    want[k0, k0] = A * A - 2 * c * A + c * c / 2
+    assert np.allclose(G2, want, rtol=0, atol=1e-14)
+
+
+def test_kspace_g2_vacuum_is_uncorrelated(orc):
+    """Wigner vacuum: ft(k) complex Gaussian with <|ft|^2> = c/2 independently
+    per k.  Normal-ordered moments vanish: n(k) -> 0 and G2(k, k') -> 0
+    (diagonal: <|z|^4> = 2 <|z|^2>^2 for a complex Gaussian)."""
+    N, B, c = 16, 40000, 0.25
+    rng = np.random.default_rng(7)
+    ft = (rng.standard_normal((N, B)) + 1j * rng.standard_normal((N, B))) * np.sqrt(c / 4)
+    frames = np.fft.ifft(ft, axis=0) * N
+    nks, G2 = orc.kspace_g2(frames, c)
+    assert np.max(abs(nks)) < 5 * (c / 2) / np.sqrt(B)
+    assert np.max(abs(G2)) < 8 * (c / 2) ** 2 / np.sqrt(B) * 2
+
+
+def test_kspace_g2_matrix_form_equals_pairwise(orc):
+    rng = np.random.default_rng(3)
+    frames = rng.standard_normal((32, 7)) + 1j * rng.standard_normal((32, 7))
+    n1, g1 = orc.kspace_g2(frames, 0.3)
+    n2, g2 = orc.kspace_g2(frames, 0.3, pairwise=False)
+    assert np.array_equal(n1, n2)
+    assert np.allclose(g1, g2, rtol=1e-13, atol=1e-13 * np.max(abs(g1)))
+    assert np.allclose(g1, g1.T, rtol=0, atol=0)
