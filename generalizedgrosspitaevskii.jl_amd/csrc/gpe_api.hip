@@ -123,7 +123,7 @@ struct gpe_handle {
   int plan_ncand = 0;
   int plan_cand[kPlanMaxCand][2] = {};
   double plan_us[kPlanMaxCand] = {};
-  // gpe_kspace_g2 scratch: |psi(k)|^2 per trajectory, nbar, n(k), G2
+  // scratch of gpe_kspace_g2 / gpe_windowed_g1 (transformed ensembles, sums)
   void* g2_buf = nullptr;
   size_t g2_cap = 0;
 };
@@ -853,29 +853,50 @@ extern "C" int gpe_set_observables(gpe_handle* h, double* n, double* n2) {
   return GPE_OK;
 }
 
-// The 2D step loop as a captured HIP graph: steps_per_graph repetitions of
-// (y-pass, x-pass) with the arguments of a step inside a save block.  Those
-// arguments are the same for every such step when no noise is drawn and the
-// envelope is static (the Philox counter in HalfStep::call is then unused),
-// so one graph, replayed once per save block, issues exactly the launches of
-// the per-step loop.  Capture always ends, also after a failed launch.
-static int capture_steps(gpe_handle* h, const XPassArgs& xa, const YPassArgs& ya, int64_t steps_per_graph,
-                         hipGraphExec_t* exec) {
+// The 2D step loop as a HIP graph: steps_per_graph repetitions of (y-pass,
+// x-pass) with the arguments of a step inside a save block.  Those arguments
+// are the same for every such step when no noise is drawn and the envelope is
+// static (the Philox counter in HalfStep::call is then unused), so one graph,
+// replayed once per save block, issues exactly the launches of the per-step
+// loop.  The graph is built node by node from described launches rather than
+// by capturing the stream: a capture in progress on one handle's stream makes
+// synchronous runtime calls from other threads (other handles, the caller's
+// own device work) fail, and distinct handles may run concurrently.
+static int build_steps_graph(gpe_handle* h, const XPassArgs& xa, const YPassArgs& ya, int64_t steps_per_graph,
+                             hipGraphExec_t* exec) {
+  LaunchDesc dx, dy;
+  const int64_t nlines = h->npts / h->n0;
+  hipError_t e = h->f64 ? gpe_run_x_f64(h->M, h->n0, xa, nlines, h->plan_w, h->stream, &dx)
+                        : gpe_run_x_f32(h->M, h->n0, xa, nlines, h->plan_w, h->stream, &dx);
+  if (e == hipSuccess)
+    e = h->f64 ? gpe_run_y_f64(h->M, h->n1, ya, h->plan_c, h->stream, &dy)
+               : gpe_run_y_f32(h->M, h->n1, ya, h->plan_c, h->stream, &dy);
+  if (e != hipSuccess) return fail(GPE_ERR_DEVICE, "gpe_solve: step graph: %s", hipGetErrorString(e));
   hipGraph_t g = nullptr;
-  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-  hipError_t e = hipSuccess;
+  HIPCHK(hipGraphCreate(&g, 0));
+  hipGraphNode_t prev = nullptr;
+  auto add = [&](LaunchDesc& d) -> hipError_t {
+    hipKernelNodeParams p;
+    memset(&p, 0, sizeof p);
+    void* args[] = {d.arg};
+    p.func = const_cast<void*>(d.func);
+    p.gridDim = d.grid;
+    p.blockDim = d.block;
+    p.sharedMemBytes = d.lds;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    const hipError_t r = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+    if (r == hipSuccess) prev = node;
+    return r;
+  };
   for (int64_t s = 0; s < steps_per_graph && e == hipSuccess; ++s) {
-    e = run_y(h, ya);
-    if (e == hipSuccess) e = run_x(h, xa);
+    e = add(dy);
+    if (e == hipSuccess) e = add(dx);
   }
-  const hipError_t e_end = hipStreamEndCapture(h->stream, &g);
-  if (e != hipSuccess || e_end != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    return fail(GPE_ERR_DEVICE, "gpe_solve: step graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e_end));
-  }
-  e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return fail(GPE_ERR_DEVICE, "gpe_solve: step graph instantiation: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return fail(GPE_ERR_DEVICE, "gpe_solve: step graph: %s", hipGetErrorString(e));
   return GPE_OK;
 }
 
@@ -1018,7 +1039,7 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     xg.flags = kInK | kEnd | kStart | kOutK;
     xg.h_end = make_half(h, rdt, 1, 1, call0);
     xg.h_start = make_half(h, rdt, 2, 0, call0);
-    rc = capture_steps(h, xg, y_args(h), sps - 1, &gexec);
+    rc = build_steps_graph(h, xg, y_args(h), sps - 1, &gexec);
     if (rc) return rc;
   }
   GraphExecGuard exec_guard{gexec, h->stream};
@@ -1156,6 +1177,48 @@ extern "C" int gpe_kspace_g2(gpe_handle* h, const void* frame, double commutator
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(nk, dnk, n * sizeof(double), hipMemcpyDefault, h->stream));
   HIPCHK(hipMemcpyAsync(g2, dg2, (size_t)n * n * sizeof(double), hipMemcpyDefault, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
+extern "C" int gpe_windowed_g1(gpe_handle* h, const void* frame, const double* w1, const double* w2, double* g1) {
+  if (!h || !frame || !w1 || !w2 || !g1) return fail(GPE_ERR_INVALID, "gpe_windowed_g1: null argument");
+  if (h->ndim != 1)
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_windowed_g1: 1D ensembles only (a 2D grid's pair matrix has prod(n)^2 entries)");
+  HIPCHK(hipSetDevice(h->d.device));
+  const int64_t n = h->n0, nb = h->batch;
+  const void* src = frame;
+  if (!is_device_ptr(frame)) {
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage[0], frame, field_bytes(h), hipMemcpyHostToDevice, h->stream));
+    src = h->stage[0];
+  }
+  // complex FP64: two transformed ensembles, the windows, the matrix
+  const size_t need = (size_t)(2 * 2 * nb * n + 2 * 2 * n + 2 * n * n) * sizeof(double);
+  int rc = ensure_buf(&h->g2_buf, &h->g2_cap, need);
+  if (rc) return rc;
+  double* x1 = (double*)h->g2_buf;
+  double* x2 = x1 + 2 * nb * n;
+  double* dw1 = x2 + 2 * nb * n;
+  double* dw2 = dw1 + 2 * n;
+  double* dg1 = dw2 + 2 * n;
+  HIPCHK(hipMemcpyAsync(dw1, w1, 2 * n * sizeof(double), hipMemcpyDefault, h->stream));
+  HIPCHK(hipMemcpyAsync(dw2, w2, 2 * n * sizeof(double), hipMemcpyDefault, h->stream));
+  KWinArgs ka;
+  ka.u = src;
+  ka.w1 = dw1;
+  ka.w2 = dw2;
+  ka.x1 = x1;
+  ka.x2 = x2;
+  ka.batch = nb;
+  ka.tw = h->tw;
+  ka.tws = (int)(h->ntw / n);
+  HIPCHK(h->f64 ? gpe_run_kwin_f64(n, ka, h->stream) : gpe_run_kwin_f32(n, ka, h->stream));
+  hipLaunchKernelGGL(kspace_g1_kernel<0>, dim3((unsigned)(n / 16), (unsigned)(n / 16)), dim3(256), 0, h->stream,
+                     (const double*)x1, (const double*)x2, n, nb, dg1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g1, dg1, 2 * (size_t)n * n * sizeof(double), hipMemcpyDefault, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return GPE_OK;
 }
@@ -1708,7 +1771,7 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
       xg.flags = kInK | kEnd | kStart | kOutK;
       xg.h_end = make_half(h, dt, 1, 1, call0);
       xg.h_start = make_half(h, dt, 2, 0, call0);
-      rc = capture_steps(h, xg, ya, check_every - 2, &gexec);
+      rc = build_steps_graph(h, xg, ya, check_every - 2, &gexec);
       if (rc) return cleanup(rc);
     }
     h->last_schedule = graph ? GPE_SCHEDULE_GRAPH : GPE_SCHEDULE_LAUNCHES;

@@ -3,17 +3,20 @@
 #include "gpe_inst.hpp"
 
 #define GPE_XCASE_f64(n)                                        \
-  case n * 4 + 1: return gpe::launch_x<double, n, 1>(a, nlines, w, s); \
-  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_x<double, (n > 2048 ? 2048 : n), 2>(a, nlines, w, s);
+  case n * 4 + 1: return gpe::launch_x<double, n, 1>(a, nlines, w, s, d); \
+  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_x<double, (n > 2048 ? 2048 : n), 2>(a, nlines, w, s, d);
 #define GPE_YCASE_f64(n)                                   \
-  case n * 4 + 1: return gpe::launch_y<double, n, 1>(a, c, s); \
-  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_y<double, (n > 2048 ? 2048 : n), 2>(a, c, s);
+  case n * 4 + 1: return gpe::launch_y<double, n, 1>(a, c, s, d); \
+  case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_y<double, (n > 2048 ? 2048 : n), 2>(a, c, s, d);
 #define GPE_1CASE_f64(n)                                    \
   case n * 4 + 1: return gpe::launch_1d<double, n, 1>(a, s); \
   case n * 4 + 2: if (n > 2048) return hipErrorInvalidValue; return gpe::launch_1d<double, (n > 2048 ? 2048 : n), 2>(a, s);
 
 #define GPE_KCASE_f64(n) \
   case n: return gpe::launch_kabs2<double, n>(a, s);
+
+#define GPE_WCASE_f64(n) \
+  case n: return gpe::launch_kwin<double, n>(a, s);
 
 #define GPE_SCASE_f64(n) \
   case n: return gpe::launch_sweep<double, n>(a, s);

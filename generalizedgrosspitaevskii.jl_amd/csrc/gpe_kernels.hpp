@@ -1021,6 +1021,95 @@ __global__ void __launch_bounds__(256) kspace_g2_kernel(const double* a, const d
   g2[kn * n + km] = (s + comp) / (double)batch - d * (comm / 2) * (nbar[km] + nbar[kn] - comm / 2);
 }
 
+// Windowed first-order correlation of a 1D ensemble frame
+// (test/windowed_ft.jl:31-49): X_j,b = fft(w_j u_b) for two windows w_1, w_2,
+// G(k1, k2) = sum_b X_1,b(k1) conj(X_2,b(k2)) / (N batch), in FFT order (the
+// reference's fftshift / ifftshift pair is a permutation with signs (-1)^k,
+// applied by the caller).
+struct KWinArgs {
+  const void* u;       // N x batch complex values
+  const double* w1;    // N complex FP64 window values
+  const double* w2;
+  double* x1;          // batch x N complex FP64, k fastest
+  double* x2;
+  int64_t batch;
+  const void* tw;
+  int tws;
+};
+
+template <typename R, int N, int W>
+__global__ void __launch_bounds__(W * (N / kE)) kspace_windowed_kernel(KWinArgs p) {
+  constexpr int T = N / kE;
+  constexpr int LS = lds_line_stride(N);
+  constexpr bool LOCAL = T <= 64;
+  extern __shared__ unsigned char smem_raw[];
+  Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int l = tid / T, t = tid % T;
+  const int64_t line_raw = (int64_t)blockIdx.x * W + l;
+  const bool valid = line_raw < p.batch;
+  const int64_t line = valid ? line_raw : p.batch - 1;
+  const Cx<R>* uc = reinterpret_cast<const Cx<R>*>(p.u) + line * N;
+  Cx<R> u[kE];
+#pragma unroll
+  for (int m = 0; m < kE; ++m) u[m] = uc[t + m * T];
+  TwReg<R, N> tr;
+  load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(p.tw), p.tws);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const double* w = j ? p.w2 : p.w1;
+    double* x = j ? p.x2 : p.x1;
+    Cx<R> v[kE];
+#pragma unroll
+    for (int m = 0; m < kE; ++m) {
+      const int i = t + m * T;
+      v[m] = u[m] * mk<R>((R)w[2 * i], (R)w[2 * i + 1]);
+    }
+    // the second transform reuses the LDS buffers of the first
+    if (j) line_sync<LOCAL>();
+    fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), lds + l * LS, 0, W * LS, t, tr);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < kE; ++m) {
+        const int64_t o = 2 * (line * N + t + m * T);
+        x[o] = (double)v[m].re;
+        x[o + 1] = (double)v[m].im;
+      }
+    }
+  }
+}
+
+// G(k1, k2) on 16 x 16 tiles, trajectory sums staged through LDS and
+// accumulated with Neumaier compensation (real and imaginary parts).
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(256) kspace_g1_kernel(const double* x1, const double* x2, int64_t n, int64_t batch,
+                                                        double* g1) {
+  __shared__ double ar[16][17], ai[16][17], br[16][17], bi[16][17];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int64_t m0 = (int64_t)blockIdx.x * 16, n0 = (int64_t)blockIdx.y * 16;
+  double sr = 0, cr = 0, si = 0, ci = 0;
+  for (int64_t b0 = 0; b0 < batch; b0 += 16) {
+    const int64_t b = b0 + ty;
+    const bool in = b < batch;
+    ar[ty][tx] = in ? x1[2 * (b * n + m0 + tx)] : 0.0;
+    ai[ty][tx] = in ? x1[2 * (b * n + m0 + tx) + 1] : 0.0;
+    br[ty][tx] = in ? x2[2 * (b * n + n0 + tx)] : 0.0;
+    bi[ty][tx] = in ? x2[2 * (b * n + n0 + tx) + 1] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      // (a + i b') (c - i d) = (a c + b' d) + i (b' c - a d)
+      neumaier_add(sr, cr, ar[j][tx] * br[j][ty] + ai[j][tx] * bi[j][ty]);
+      neumaier_add(si, ci, ai[j][tx] * br[j][ty] - ar[j][tx] * bi[j][ty]);
+    }
+    __syncthreads();
+  }
+  const double scale = 1.0 / ((double)n * (double)batch);
+  const int64_t o = 2 * ((n0 + ty) * n + m0 + tx);
+  g1[o] = (sr + cr) * scale;
+  g1[o + 1] = (si + ci) * scale;
+}
+
 struct ResidualArgs {
   const void* a[kMaxM];  // u_{n+1} (real space)
   const void* b[kMaxM];  // u_n

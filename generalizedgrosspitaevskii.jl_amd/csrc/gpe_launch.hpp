@@ -50,18 +50,34 @@ __host__ __device__ constexpr int y_rule_cols(int N, int M, bool f64, int64_t nx
              : y_cols_per_wg(N);
 }
 
+// A launch described instead of issued: kernel, shape and a copy of its
+// argument struct, for adding the launch to a HIP graph as a kernel node.
+struct LaunchDesc {
+  const void* func = nullptr;
+  dim3 grid, block;
+  unsigned lds = 0;
+  alignas(16) unsigned char arg[512];
+};
+
 }  // namespace gpe
 
 // Pass launchers: w / c rows / columns per workgroup, 0 for the built-in
 // shape (x_lines_per_wg, y_rule_cols); other values must satisfy
-// x_plan_valid / y_plan_valid (hipErrorInvalidValue otherwise).
-hipError_t gpe_run_x_f32(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s);
-hipError_t gpe_run_x_f64(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s);
-hipError_t gpe_run_y_f32(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s);
-hipError_t gpe_run_y_f64(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s);
+// x_plan_valid / y_plan_valid (hipErrorInvalidValue otherwise).  With d
+// non-null the x / y launchers fill *d and launch nothing.
+hipError_t gpe_run_x_f32(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s,
+                         gpe::LaunchDesc* d = nullptr);
+hipError_t gpe_run_x_f64(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s,
+                         gpe::LaunchDesc* d = nullptr);
+hipError_t gpe_run_y_f32(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s,
+                         gpe::LaunchDesc* d = nullptr);
+hipError_t gpe_run_y_f64(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s,
+                         gpe::LaunchDesc* d = nullptr);
 hipError_t gpe_run_sweep_f32(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
 hipError_t gpe_run_sweep_f64(int64_t N, const gpe::SweepArgs& a, hipStream_t s);
 hipError_t gpe_run_1d_f32(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);
 hipError_t gpe_run_1d_f64(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s);
 hipError_t gpe_run_kabs2_f32(int64_t N, const gpe::KAbs2Args& a, hipStream_t s);
 hipError_t gpe_run_kabs2_f64(int64_t N, const gpe::KAbs2Args& a, hipStream_t s);
+hipError_t gpe_run_kwin_f32(int64_t N, const gpe::KWinArgs& a, hipStream_t s);
+hipError_t gpe_run_kwin_f64(int64_t N, const gpe::KWinArgs& a, hipStream_t s);

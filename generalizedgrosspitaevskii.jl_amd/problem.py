@@ -658,16 +658,11 @@ class StrangSplittingIterator:
         L.check(self._lib.gpe_last_solve_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
-    def kspace_correlations(self, frame, commutator):
-        """n(k) and G2(k, k') of one 1D ensemble frame, computed on the
-        device as examples/truncated_wigner.jl:110-165 does on the host:
-        ft = fft(u_b)/N per trajectory, nks = mean |ft|^2 - c/2,
-        G2[k, k'] = mean_b f(ft(k), ft(k'), k == k', c).  frame: (N, batch)
-        field of this handle's dtype (numpy or device tensor, e.g.
-        sol[0][..., f]); commutator: 1/L in the example.  Returns (nks, G2)
-        in FFT order (apply np.fft.fftshift for the example's axes)."""
+    def _ensemble_frame(self, frame, who):
+        """(pointer, keep-alive) of one 1D ensemble frame in this handle's
+        dtype and layout."""
         if self.prob.N != 1:
-            raise NotImplementedError("kspace_correlations: 1D ensembles only")
+            raise NotImplementedError(f"{who}: 1D ensembles only")
         n = self.spatial[0]
         want = str(self.prob.u0[0].dtype).replace("torch.", "")
         if _is_torch(frame) and frame.is_cuda:
@@ -679,11 +674,43 @@ class StrangSplittingIterator:
             f = np.asfortranarray(np.asarray(frame).astype(want))
             ptr = f.ctypes.data
         if int(np.prod(tuple(f.shape))) != n * int(np.prod(self.shape[1:] or (1,))):
-            raise ValueError(f"kspace_correlations: frame of shape {tuple(f.shape)} does not match {self.shape}")
+            raise ValueError(f"{who}: frame of shape {tuple(f.shape)} does not match {self.shape}")
+        return ptr, f
+
+    def kspace_correlations(self, frame, commutator):
+        """n(k) and G2(k, k') of one 1D ensemble frame, computed on the
+        device as examples/truncated_wigner.jl:110-165 does on the host:
+        ft = fft(u_b)/N per trajectory, nks = mean |ft|^2 - c/2,
+        G2[k, k'] = mean_b f(ft(k), ft(k'), k == k', c).  frame: (N, batch)
+        field of this handle's dtype (numpy or device tensor, e.g.
+        sol[0][..., f]); commutator: 1/L in the example.  Returns (nks, G2)
+        in FFT order (apply np.fft.fftshift for the example's axes)."""
+        ptr, keep = self._ensemble_frame(frame, "kspace_correlations")
+        n = self.spatial[0]
         nk = np.empty(n)
         G2 = np.empty((n, n), order="F")
         L.check(self._lib.gpe_kspace_g2(self.h, ptr, float(commutator), nk.ctypes.data, G2.ctypes.data))
+        del keep
         return nk, np.ascontiguousarray(G2)
+
+    def windowed_correlation(self, frame, w1, w2):
+        """The windowed first-order correlation of test/windowed_ft.jl:31-49
+        on the device: ft_j = ifftshift(fft(fftshift(w_j .* u_b))) per
+        trajectory and g1[i, j] = sum_b conj(ft_2[j, b]) ft_1[i, b] / (N batch).
+        w1, w2: the windows' values at the grid indices (length N, real or
+        complex).  Returns the N x N complex matrix in the reference's layout."""
+        ptr, keep = self._ensemble_frame(frame, "windowed_correlation")
+        n = self.spatial[0]
+        wa = np.ascontiguousarray(np.broadcast_to(np.asarray(w1, dtype=np.complex128), (n,)))
+        wb = np.ascontiguousarray(np.broadcast_to(np.asarray(w2, dtype=np.complex128), (n,)))
+        g = np.empty((n, n), dtype=np.complex128, order="F")
+        L.check(self._lib.gpe_windowed_g1(self.h, ptr, wa.ctypes.data, wb.ctypes.data, g.ctypes.data))
+        del keep
+        # the library returns FFT order; the shift pair maps index k to
+        # (k + N/2) mod N with the sign (-1)^k of the half-length shift
+        idx = (np.arange(n) + n // 2) % n
+        sgn = np.where(idx % 2 == 0, 1.0, -1.0)
+        return g[np.ix_(idx, idx)] * sgn[:, None] * sgn[None, :]
 
     def get_field(self):
         """The handle's current real-space field (host numpy arrays)."""

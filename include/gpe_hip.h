@@ -202,6 +202,17 @@ int gpe_get_plan(gpe_handle* h, int32_t* x_rows, int32_t* y_cols, int32_t* cand,
  * memory.  1D handles only. */
 int gpe_kspace_g2(gpe_handle* h, const void* frame, double commutator, double* nk, double* g2);
 
+/* Windowed first-order correlation of a 1D ensemble frame, the `correlation`
+ * of test/windowed_ft.jl:31-49: with X_j,b = fft(w_j .* u_b) (unnormalised,
+ * FFT order) for the windows w1, w2 (N complex FP64 values each, host or
+ * device) and the trajectories u_b of `frame` (N x batch values, field
+ * precision, host or device),
+ *   g1[2 (k2 N + k1) + {0, 1}] = sum_b X_1,b(k1) conj(X_2,b(k2)) / (N batch)
+ * (complex FP64, host or device; compensated sums).  The reference's
+ * fftshift / ifftshift pair maps this to its layout by the index shift
+ * k -> (k + N/2) mod N and the sign (-1)^k per index.  1D handles only. */
+int gpe_windowed_g1(gpe_handle* h, const void* frame, const double* w1, const double* w2, double* g1);
+
 /* Copy the field after the last solve (real space) into u[c]. */
 int gpe_get_field(gpe_handle* h, void* const* u);
 

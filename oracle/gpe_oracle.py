@@ -21,6 +21,7 @@ It restates, in numpy complex128 arithmetic, what the reference computes for
 * ``evaluate_pump``                <- src/misc.jl:22-42
 * FFT over the spatial dims        <- src/misc.jl:53-64 (FFTW plan_fft / inv)
 * ``kspace_g2`` (n(k), G2(k,k'))    <- examples/truncated_wigner.jl:110-165
+* ``windowed_correlation``, ``analytic_commutation`` <- test/windowed_ft.jl:31-59
 
 User closures follow the reference's calling conventions, evaluated
 vectorised over the grid instead of point by point:
@@ -509,3 +510,28 @@ def kspace_g2(frames, commutator, pairwise=True):
             d = 1.0 if m == n else 0.0
             G2[m, n] = np.mean(a[m] * a[n] - (1 + d) * c / 2 * (a[m] + a[n] - c / 2))
     return nks, G2
+
+
+def windowed_correlation(frames, w1, w2):
+    """The `correlation` of test/windowed_ft.jl:31-49: per trajectory (axis 1)
+    ft_j = ifftshift(fft(fftshift(frames .* w_j, axis 0), axis 0), axis 0) and
+    g1[i, j] = sum_b conj(ft_2[j, b]) ft_1[i, b] / (N B), with w_j the windows'
+    values at the grid indices."""
+    u = np.asarray(frames, dtype=np.complex128)
+    N, B = u.shape
+    fts = []
+    for w in (w1, w2):
+        x = u * np.asarray(w, dtype=np.complex128)[:, None]
+        fts.append(np.fft.ifftshift(np.fft.fft(np.fft.fftshift(x, axes=0), axis=0), axes=0))
+    return fts[0] @ fts[1].conj().T / (N * B)
+
+
+def analytic_commutation(L, N, w1, w2):
+    """The vacuum expectation of test/windowed_ft.jl:51-59 for windows given
+    as functions of r: sum_r exp(i (k - k') r) w1(r) conj(w2(r)) / (r_N - r_1) / 2
+    on r = -L/2 + j L/N and k = -pi/L + 2 pi m / L."""
+    rs = -L / 2 + np.arange(N) * (L / N)
+    ks = -np.pi / L + 2 * np.pi / L * np.arange(N)
+    w = np.asarray(w1(rs), dtype=np.complex128) * np.conj(np.asarray(w2(rs), dtype=np.complex128))
+    ph = np.exp(1j * (ks[:, None, None] - ks[None, :, None]) * rs[None, None, :])
+    return (ph * w[None, None, :]).sum(axis=2) / (rs[-1] - rs[0]) / 2

@@ -65,3 +65,55 @@ def test_kspace_g2_device_frame_and_twa_vacuum(ggp, orc):
     # free evolution keeps the vacuum: n(k) ~ 0, G2 ~ 0 (scale (c/2)^2 / sqrt(B))
     assert np.max(abs(nk_h)) < 6 * (ck / 2) / np.sqrt(B)
     assert np.max(abs(G2_h)) < 20 * (ck / 2) ** 2 / np.sqrt(B)
+
+
+@pytest.mark.parametrize("N,B,dtype,tol", [(16, 21, np.complex128, 1e-12), (64, 300, np.complex128, 1e-12),
+                                           (512, 17, np.complex64, 1e-5), (2048, 9, np.complex128, 1e-12)])
+def test_windowed_correlation_matches_oracle(ggp, orc, N, B, dtype, tol):
+    rng = np.random.default_rng(2 * N + B)
+    frames = ((rng.standard_normal((N, B)) + 1j * rng.standard_normal((N, B))) * 0.3).astype(dtype)
+    x = np.arange(N) / N
+    w1 = np.exp(-((x - 0.4) / 0.2) ** 2)
+    w2 = np.exp(-((x - 0.6) / 0.3) ** 2) * np.exp(0.7j * x)
+    it = _handle(ggp, N, B, dtype)
+    g = it.windowed_correlation(frames, w1, w2)
+    it.close()
+    g_o = orc.windowed_correlation(frames.astype(np.complex128), w1, w2)
+    e = _rel(g, g_o)
+    print(f"windowed N={N} B={B} {np.dtype(dtype).name}: {e:.2e}")
+    assert g.shape == (N, N) and e <= tol
+
+
+def test_reference_windowed_ft_vacuum(ggp, orc):
+    """test/windowed_ft.jl end to end: the damped, noise-driven polariton
+    field relaxes to the vacuum, and the windowed correlation of 10^4
+    trajectories (computed on the device) matches the analytic commutator
+    within the reference's rtol 7e-2 for its nine window pairs."""
+    L, N, B = 20.0, 64, 10 ** 4
+    dL = L / N
+    hb = 0.6582
+    gam, m, d0 = 0.1 / hb, hb ** 2 / 2.5, 0.49 / hb
+    param = dict(hb=hb, gam=gam, m=m, d0=d0, dL=dL)
+    u0 = (np.zeros((N, B), complex),)
+    prob = ggp.GrossPitaevskiiProblem(
+        u0, (L,), param=param,
+        dispersion=lambda k, p: -1j * p["gam"] / 2 + p["hb"] * k[0] ** 2 / (2 * p["m"]) - p["d0"],
+        position_noise_func=lambda u, r, p: np.sqrt(p["gam"] / 2 / p["dL"]),
+        noise_prototype=(np.empty((N, B), complex),))
+    it = ggp.init(prob, ggp.StrangSplitting(), (0.0, 200.0), dt=4.0, nsaves=1, save_start=False,
+                  rng=ggp.PhiloxRNG(1234))
+    ts, res = it.solve()
+    frame = res[0][..., 0]
+    rs = -L / 2 + np.arange(N) * dL
+    worst = 0.0
+    for x0 in (-1, 0, 1):
+        for w in (3, 4, 5):
+            f1 = lambda r, x0=x0, w=w: np.exp(-(r - x0) ** 2 / w ** 2)
+            f2 = lambda r, x0=x0, w=w: np.exp(-(r + x0) ** 2 / w ** 2)
+            corr = it.windowed_correlation(frame, f1(rs), f2(rs))
+            an = orc.analytic_commutation(L, N, f1, f2)
+            err = np.linalg.norm(corr - an) / max(np.linalg.norm(corr), np.linalg.norm(an))
+            worst = max(worst, err)
+            assert err <= 7e-2, (x0, w, err)
+    it.close()
+    print(f"windowed_ft vacuum: worst relative deviation {worst:.3f} (bar 7e-2)")

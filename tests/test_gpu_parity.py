@@ -425,3 +425,49 @@ def test_measured_plan_matches_estimate_bit_for_bit(ggp, case):
     with pytest.raises(ggp.GpeError):
         it.set_plan(3, 0)
     it.close()
+
+
+def test_step_graphs_on_concurrent_handles(ggp):
+    """Distinct handles on several host threads, each building and replaying
+    step graphs (solve and steady state) while the main thread keeps the
+    default stream busy with synchronous copies: every run succeeds and equals
+    its sequential twin bit for bit (the graphs are built node by node, so no
+    stream is ever in capture mode)."""
+    import threading
+
+    import torch
+
+    u0, lengths = c1_inputs(32)
+    probs = [ggp.GrossPitaevskiiProblem(u0, lengths, dispersion=disp_kin,
+                                        nonlinearity=lambda u, p, g=g: g * abs(u[0]) ** 2) for g in (-1.0, -2.0, 0.5, 1.5)]
+
+    def run(prob):
+        it = ggp.init(prob, ggp.StrangSplitting(), (0, 0.4), dt=0.005, nsaves=10)
+        it.set_schedule("graph")
+        ts, res = it.solve()
+        u, info = it.steady_state(tol=1e-300, check_every=8, max_steps=64)
+        it.close()
+        return res[0], u[0], info["history"]
+
+    seq = [run(p) for p in probs]
+    out = [None] * len(probs)
+    errs = []
+
+    def worker(i):
+        try:
+            for _ in range(3):
+                out[i] = run(probs[i])
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(probs))]
+    for t in threads:
+        t.start()
+    x = torch.zeros(1 << 20, device="cuda")
+    while any(t.is_alive() for t in threads):
+        x.cpu()  # synchronous device-to-host copies on the default stream
+    for t in threads:
+        t.join()
+    assert not errs, errs
+    for (a1, b1, h1), (a2, b2, h2) in zip(seq, out):
+        assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and np.array_equal(h1, h2)

@@ -363,3 +363,36 @@ def test_kspace_g2_matrix_form_equals_pairwise(orc):
     assert np.array_equal(n1, n2)
     assert np.allclose(g1, g2, rtol=1e-13, atol=1e-13 * np.max(abs(g1)))
     assert np.allclose(g1, g1.T, rtol=0, atol=0)
+
+
+# ------------------------------------------------ windowed correlation
+def test_windowed_correlation_plane_wave_layout(orc):
+    """One trajectory exp(2 pi i k0 n / N) with windows 1 and i: after the
+    fftshift / ifftshift pair the only entry sits at i0 = (k0 + N/2) mod N and
+    equals conj(i N) N / N = -i N, which fixes the output shift and which
+    factor is conjugated (the vacuum check below is blind to both)."""
+    N, k0 = 16, 5
+    u = np.exp(2j * np.pi * k0 * np.arange(N) / N)[:, None]
+    g = orc.windowed_correlation(u, np.ones(N), 1j * np.ones(N))
+    i0 = (k0 + N // 2) % N
+    want = np.zeros((N, N), complex)
+    want[i0, i0] = -1j * N
+    assert np.allclose(g, want, rtol=0, atol=1e-12)
+
+
+def test_windowed_correlation_vacuum_matches_analytic(orc):
+    """test/windowed_ft.jl's check with vacuum samples drawn directly:
+    [psi(r), psi^dagger(r')] = delta N/L, so <|psi|^2>_W = N/(2L); the
+    windowed correlation matches the analytic commutator within the
+    reference's rtol 7e-2 for its nine window pairs."""
+    L, N, B = 20.0, 64, 10000
+    rng = np.random.default_rng(1)
+    u = (rng.standard_normal((N, B)) + 1j * rng.standard_normal((N, B))) * np.sqrt(N / L / 4)
+    rs = -L / 2 + np.arange(N) * (L / N)
+    for x0 in (-1, 0, 1):
+        for w in (3, 4, 5):
+            f1 = lambda r, x0=x0, w=w: np.exp(-(r - x0) ** 2 / w ** 2)
+            f2 = lambda r, x0=x0, w=w: np.exp(-(r + x0) ** 2 / w ** 2)
+            corr = orc.windowed_correlation(u, f1(rs), f2(rs))
+            an = orc.analytic_commutation(L, N, f1, f2)
+            assert np.linalg.norm(corr - an) <= 7e-2 * max(np.linalg.norm(corr), np.linalg.norm(an))
