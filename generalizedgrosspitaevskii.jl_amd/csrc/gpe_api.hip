@@ -913,8 +913,8 @@ struct GraphExecGuard {
   }
 };
 
-// Whether a 2D solve replays a step graph.  The capture costs about as much
-// host time as issuing the launches of the captured steps once, and each
+// Whether a 2D solve replays a step graph.  Building it costs about as much
+// host time as issuing the launches of its steps once, and each
 // replay saves ~0.3-0.9 us per step on the device (measured on MI355X from
 // 64x64 to 512x512), so AUTO replays when there are at least kGraphMinSaves
 // save blocks.
@@ -1761,7 +1761,7 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
     XPassArgs xa = x_args(h);
     const YPassArgs ya = y_args(h);
     // the steps of a check block before the two that write the compared
-    // fields are identical without noise: one captured graph replays them
+    // fields are identical without noise: one prebuilt graph replays them
     hipGraphExec_t gexec = nullptr;
     GraphExecGuard exec_guard{gexec, h->stream};
     const bool graph =

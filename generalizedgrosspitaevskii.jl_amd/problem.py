@@ -502,7 +502,7 @@ class StrangSplittingIterator:
 
     def set_schedule(self, schedule):
         """How solve issues the 2D step loop: "launches" (two kernel launches
-        per step), "graph" (the steps of a save block captured once as a HIP
+        per step), "graph" (the steps of a save block built once into a HIP
         graph and replayed per block; no noise, static envelope) or "auto"
         (the graph from 8 save blocks on).  The results are identical bit for
         bit.  No reference counterpart (the reference steps on the host)."""

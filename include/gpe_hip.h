@@ -145,7 +145,7 @@ int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double t1, double
  * one broadcast kernel per operation (src/fixed_time_stepping.jl:40-46,
  * src/strang_splitting.jl:86-90); the choice has no counterpart there.
  *   GPE_SCHEDULE_LAUNCHES: two launches per step (column pass, row pass).
- *   GPE_SCHEDULE_GRAPH:    the steps inside each save block are captured once
+ *   GPE_SCHEDULE_GRAPH:    the steps inside each save block are built once
  *                          per solve as a HIP graph and replayed per block.
  *                          Needs no noise, a static envelope and >= 2 steps
  *                          per save (GPE_ERR_UNSUPPORTED otherwise).

@@ -55,7 +55,7 @@ def test_steady_state_converges(ggp):
 
 def test_steady_state_graph_replay_is_bit_identical(ggp):
     """Without noise the steps of a check block before the two compared
-    fields replay as one captured graph; residual history, step count and
+    fields replay as one prebuilt graph; residual history, step count and
     final field equal the launch-per-step run bit for bit."""
     N, L = 64, 25.0
     u0 = (np.zeros((N, N), complex), np.zeros((N, N), complex))
