@@ -100,6 +100,8 @@ def gpu_run(g, torch, dtype, steps, warmup, dt=0.05, plan=None):
     it_w = make_iter(warmup)
     if plan is not None:
         it_w.set_plan(*plan)
+    else:
+        it_w.set_planner("measure")
     it_w.solve()  # measures the launch plan unless one is given
     plan = it_w.plan
     it_w.close()
@@ -156,6 +158,7 @@ def twa_run(g, torch, steps, batch=1024, n=256, dtype="complex64"):
     nsaves = max(1, steps // SAVE_EVERY)
     it = g.init(prob, g.StrangSplitting(), (0.0, steps * dt), dt=dt, nsaves=nsaves, rng=g.PhiloxRNG(0x5EED0003))
     it_w = g.init(prob, g.StrangSplitting(), (0.0, 20 * dt), dt=dt, nsaves=1, rng=g.PhiloxRNG(1))
+    it_w.set_planner("measure")
     it_w.solve(save_frames=False, ensemble_moments=True)  # also measures the launch plan
     plan = it_w.plan
     it.set_plan(plan["x_rows"], plan["y_cols"])  # reuse it (FFTW-wisdom style)

@@ -41,6 +41,7 @@ GPE_SCHEDULE_GRAPH = 2
 
 GPE_PLAN_ESTIMATE = 0
 GPE_PLAN_MEASURE = 1
+GPE_PLAN_AUTO = 2
 
 
 class ProblemDesc(ctypes.Structure):

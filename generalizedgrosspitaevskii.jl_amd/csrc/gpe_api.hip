@@ -10,6 +10,9 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -116,7 +119,7 @@ struct gpe_handle {
   // launch shapes of the 2D passes (rows / columns per workgroup; 0: the
   // built-in rule), how they were chosen, and the last measurement: the
   // (rows, columns) pairs timed and their us per step
-  int planner = GPE_PLAN_MEASURE;
+  int planner = GPE_PLAN_AUTO;
   int plan_w = 0, plan_c = 0;
   bool plan_fixed = false;
   uint64_t plan_key = 0;  // terms the measured plan was timed with (0: none)
@@ -723,12 +726,41 @@ static int time_launch(gpe_handle* h, const std::function<hipError_t()>& fn, dou
   return GPE_OK;
 }
 
+// Measured plans of this process, keyed by what the timing depends on
+// (device, grid, components, precision, batch, terms), so a new handle of a
+// shape measured before reuses the result, as FFTW keeps its wisdom.
+using PlanKey = std::tuple<int, int64_t, int64_t, int, int, int64_t, uint64_t>;
+static std::mutex g_plan_mu;
+static std::map<PlanKey, std::pair<int, int>> g_plans;
+
+// A measurement times 6 shape pairs x 2 rounds x ~2 ms; it is skipped (the
+// built-in shapes are used, nothing is cached) for runs of fewer steps,
+// whose whole duration it would exceed several times.
+constexpr int64_t kPlanMinSteps = 1000;
+
 // Called with the tables prepared and before the start field is copied in:
-// the candidates run in place on a zeroed field.
-static int ensure_plan(gpe_handle* h, double dt) {
-  if (h->ndim != 2 || h->planner != GPE_PLAN_MEASURE || h->plan_fixed) return GPE_OK;
+// the candidates run in place on a zeroed field.  nsteps: the steps the
+// caller is about to run.
+static int ensure_plan(gpe_handle* h, double dt, int64_t nsteps) {
+  if (h->ndim != 2 || h->planner == GPE_PLAN_ESTIMATE || h->plan_fixed) return GPE_OK;
   const uint64_t key = plan_key_of(h);
   if (h->plan_key == key) return GPE_OK;
+  const PlanKey pk{h->d.device, h->n0, h->n1, h->M, h->f64 ? 1 : 0, h->batch, key};
+  if (h->planner == GPE_PLAN_AUTO) {
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    auto it = g_plans.find(pk);
+    if (it != g_plans.end()) {
+      h->plan_w = it->second.first;
+      h->plan_c = it->second.second;
+      h->plan_key = key;
+      h->plan_ncand = 0;
+      return GPE_OK;
+    }
+  }
+  if (h->planner == GPE_PLAN_AUTO && nsteps < kPlanMinSteps) {
+    h->plan_w = h->plan_c = 0;
+    return GPE_OK;
+  }
   for (int c = 0; c < h->M; ++c) HIPCHK(hipMemsetAsync(h->u[c], 0, field_bytes(h), h->stream));
   XPassArgs xa = x_args(h);
   xa.flags = kInK | kEnd | kStart | kOutK;
@@ -781,12 +813,14 @@ static int ensure_plan(gpe_handle* h, double dt) {
   h->plan_w = best_w;
   h->plan_c = best_c;
   h->plan_key = key;
+  std::lock_guard<std::mutex> lock(g_plan_mu);
+  g_plans[pk] = {best_w, best_c};
   return GPE_OK;
 }
 
 extern "C" int gpe_set_planner(gpe_handle* h, int32_t planner) {
   if (!h) return fail(GPE_ERR_INVALID, "gpe_set_planner: null handle");
-  if (planner != GPE_PLAN_ESTIMATE && planner != GPE_PLAN_MEASURE)
+  if (planner != GPE_PLAN_ESTIMATE && planner != GPE_PLAN_MEASURE && planner != GPE_PLAN_AUTO)
     return fail(GPE_ERR_INVALID, "gpe_set_planner: planner=%d", planner);
   h->planner = planner;
   h->plan_fixed = false;
@@ -964,7 +998,7 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
                 "gpe_solve: the graph schedule needs a 2D grid, no noise, a static envelope and >= 2 steps per save");
   rc = prepare_tables(h, rdt);
   if (rc) return rc;
-  rc = ensure_plan(h, rdt);
+  rc = ensure_plan(h, rdt, S);
   if (rc) return rc;
   const size_t fb = field_bytes(h);
   for (int c = 0; c < h->M; ++c) HIPCHK(hipMemcpyAsync(h->u[c], u0[c], fb, hipMemcpyDefault, h->stream));
@@ -1710,7 +1744,7 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
   HIPCHK(hipSetDevice(h->d.device));
   int rc = prepare_tables(h, dt);
   if (rc) return rc;
-  rc = ensure_plan(h, dt);
+  rc = ensure_plan(h, dt, max_steps);
   if (rc) return rc;
   const size_t fb = field_bytes(h);
   SteadyBufs& sb = h->steady;

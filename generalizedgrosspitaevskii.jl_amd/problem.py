@@ -510,14 +510,16 @@ class StrangSplittingIterator:
             raise ValueError(f"schedule must be one of {sorted(self.SCHEDULES)}")
         L.check(self._lib.gpe_set_schedule(self.h, self.SCHEDULES[schedule]))
 
-    PLANNERS = {"estimate": L.GPE_PLAN_ESTIMATE, "measure": L.GPE_PLAN_MEASURE}
+    PLANNERS = {"estimate": L.GPE_PLAN_ESTIMATE, "measure": L.GPE_PLAN_MEASURE, "auto": L.GPE_PLAN_AUTO}
 
     def set_planner(self, planner):
         """Launch-shape planner of the 2D passes, the counterpart of the FFTW
         planner flags behind get_fft_plans (src/misc.jl:53-58): "estimate"
-        (built-in shapes) or "measure" (the default: the next solve first
-        times every valid shape on this handle and keeps the fastest).
-        Results are identical under every plan."""
+        (built-in shapes), "measure" (the next solve first times every valid
+        shape pair on this handle and keeps the fastest) or "auto" (the
+        default: a plan measured before in this process for the same shape
+        and terms, else "measure" for runs of >= 1000 steps, else
+        "estimate").  Results are identical under every plan."""
         if planner not in self.PLANNERS:
             raise ValueError(f"planner must be one of {sorted(self.PLANNERS)}")
         L.check(self._lib.gpe_set_planner(self.h, self.PLANNERS[planner]))

@@ -164,17 +164,21 @@ int gpe_get_schedule(gpe_handle* h, int32_t* schedule);
 /* Planner of the 2D passes' launch shapes, the counterpart of the FFTW
  * planner behind get_fft_plans (src/misc.jl:53-58, FFTW.jl's default flags):
  *   GPE_PLAN_ESTIMATE: the built-in shapes.
- *   GPE_PLAN_MEASURE:  (default) the next gpe_solve / gpe_steady_state of a
- *                      2D handle first times whole steps for every valid
- *                      pair of row-pass shape (rows per workgroup) and
- *                      column-pass shape (columns per workgroup) on this
- *                      handle, with its tables and terms, and keeps the
- *                      fastest pair; it measures again when the terms that
- *                      select kernel code (noise, nonlinearity, dispersion,
- *                      potential or pump kind) change.
+ *   GPE_PLAN_MEASURE:  the next gpe_solve / gpe_steady_state of a 2D handle
+ *                      first times whole steps for every valid pair of
+ *                      row-pass shape (rows per workgroup) and column-pass
+ *                      shape (columns per workgroup) on this handle, with its
+ *                      tables and terms, and keeps the fastest pair; it
+ *                      measures again when the terms that select kernel code
+ *                      (noise, nonlinearity, dispersion, potential or pump
+ *                      kind) change.  Measured plans are kept for the process
+ *                      (per device, shape, precision, batch and terms).
+ *   GPE_PLAN_AUTO:     (default) a plan the process measured before for the
+ *                      same key; else MEASURE when the call runs >= 1000
+ *                      steps (the measurement takes ~25 ms); else ESTIMATE.
  * Setting the planner drops any plan fixed by gpe_set_plan.  No shape changes
  * the arithmetic: results are identical bit for bit under every plan. */
-enum gpe_planner { GPE_PLAN_ESTIMATE = 0, GPE_PLAN_MEASURE = 1 };
+enum gpe_planner { GPE_PLAN_ESTIMATE = 0, GPE_PLAN_MEASURE = 1, GPE_PLAN_AUTO = 2 };
 int gpe_set_planner(gpe_handle* h, int32_t planner);
 
 /* Fix the launch shapes (e.g. a plan measured before, the counterpart of

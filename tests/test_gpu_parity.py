@@ -471,3 +471,34 @@ def test_step_graphs_on_concurrent_handles(ggp):
     assert not errs, errs
     for (a1, b1, h1), (a2, b2, h2) in zip(seq, out):
         assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and np.array_equal(h1, h2)
+
+
+def test_auto_planner_measures_long_runs_once_per_process(ggp):
+    """AUTO (the default) keeps the built-in shapes for a short run, measures
+    for a run of >= 1000 steps, and a second handle of the same shape and terms
+    reuses that plan without measuring; the frames agree bit for bit."""
+    rng = np.random.default_rng(5)
+    u0 = (np.asfortranarray(rng.standard_normal((16, 32, 3)) + 1j * rng.standard_normal((16, 32, 3))) * 0.2,)
+    prob = ggp.GrossPitaevskiiProblem(u0, (4.0, 8.0), dispersion=disp_kin,
+                                      nonlinearity=lambda u, p: 0.7 * abs(u[0]) ** 2)
+    it = ggp.init(prob, ggp.StrangSplitting(), (0, 0.5), dt=0.01, nsaves=5)  # 50 steps
+    short = it.solve()
+    assert it.plan["measured_us"] == {}
+    it.close()
+    it = ggp.init(prob, ggp.StrangSplitting(), (0, 10.0), dt=0.01, nsaves=10)  # 1000 steps
+    it.solve()
+    first = it.plan
+    assert first["measured_us"] and (first["x_rows"], first["y_cols"]) == min(first["measured_us"],
+                                                                                key=first["measured_us"].get)
+    it.close()
+    it = ggp.init(prob, ggp.StrangSplitting(), (0, 10.0), dt=0.01, nsaves=10)
+    it.solve()
+    again = it.plan
+    it.close()
+    assert again["measured_us"] == {} and (again["x_rows"], again["y_cols"]) == (first["x_rows"], first["y_cols"])
+    it = ggp.init(prob, ggp.StrangSplitting(), (0, 0.5), dt=0.01, nsaves=5)
+    it.solve()  # a short run now reuses the kept plan
+    assert (it.plan["x_rows"], it.plan["y_cols"]) == (first["x_rows"], first["y_cols"])
+    res = it.solve()
+    it.close()
+    assert np.array_equal(short[1][0], res[1][0])
