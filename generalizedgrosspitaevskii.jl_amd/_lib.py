@@ -95,6 +95,7 @@ SIGNATURES = {
     "gpe_get_plan": (ctypes.c_int, [P, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32), D]),
     "gpe_kspace_g2": (ctypes.c_int, [P, P, ctypes.c_double, P, P]),
     "gpe_windowed_g1": (ctypes.c_int, [P, P, P, P, P]),
+    "gpe_density_g2_x": (ctypes.c_int, [P, P, ctypes.c_double, P, P]),
     "gpe_get_field": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "gpe_last_solve_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
     "gpe_advance": (ctypes.c_int, [P, I64, ctypes.POINTER(ctypes.c_float)]),

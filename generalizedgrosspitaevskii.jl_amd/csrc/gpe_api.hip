@@ -1257,6 +1257,42 @@ extern "C" int gpe_windowed_g1(gpe_handle* h, const void* frame, const double* w
   return GPE_OK;
 }
 
+extern "C" int gpe_density_g2_x(gpe_handle* h, const void* frame, double commutator, double* g2r, double* nphys) {
+  if (!h || !frame || !g2r || !nphys) return fail(GPE_ERR_INVALID, "gpe_density_g2_x: null argument");
+  if (!(commutator >= 0) || !std::isfinite(commutator))
+    return fail(GPE_ERR_INVALID, "gpe_density_g2_x: commutator=%g", commutator);
+  HIPCHK(hipSetDevice(h->d.device));
+  const int64_t n = h->n0, nlines = h->npts / h->n0;
+  const void* src = frame;
+  if (!is_device_ptr(frame)) {
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage[0], frame, field_bytes(h), hipMemcpyHostToDevice, h->stream));
+    src = h->stage[0];
+  }
+  const int nblk = gpe_rowcorr_blocks(n, nlines);
+  const size_t need = (size_t)(nblk * (n + 1) + n + 1) * sizeof(double);
+  int rc = ensure_buf(&h->g2_buf, &h->g2_cap, need);
+  if (rc) return rc;
+  double* partial = (double*)h->g2_buf;
+  double* dg2 = partial + (size_t)nblk * (n + 1);
+  double* dnp = dg2 + n;
+  RowCorrArgs ra;
+  ra.u = src;
+  ra.nlines = nlines;
+  ra.partial = partial;
+  ra.tw = h->tw;
+  ra.tws = (int)(h->ntw / n);
+  HIPCHK(h->f64 ? gpe_run_rowcorr_f64(n, ra, nblk, h->stream) : gpe_run_rowcorr_f32(n, ra, nblk, h->stream));
+  hipLaunchKernelGGL(row_autocorr_final_kernel<0>, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, h->stream,
+                     (const double*)partial, nblk, n, nlines, commutator, dg2, dnp);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g2r, dg2, n * sizeof(double), hipMemcpyDefault, h->stream));
+  HIPCHK(hipMemcpyAsync(nphys, dnp, sizeof(double), hipMemcpyDefault, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
 extern "C" int gpe_get_field(gpe_handle* h, void* const* u) {
   if (!h || !u) return fail(GPE_ERR_INVALID, "gpe_get_field: null argument");
   HIPCHK(hipSetDevice(h->d.device));

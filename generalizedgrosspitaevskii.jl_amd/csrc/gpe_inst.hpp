@@ -138,6 +138,29 @@ static hipError_t launch_kwin(const KWinArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Workgroups of row_autocorr_kernel: enough to fill the chip, each looping
+// over its share of the lines.
+constexpr int kRowCorrBlocks = 1024;
+
+template <typename R, int N>
+static hipError_t launch_rowcorr(const RowCorrArgs& a, int nblk, hipStream_t s) {
+  constexpr int W = x_lines_per_wg(N);
+  constexpr int T = N / kE;
+  const size_t lds = xlds_bytes<R, N, 1, W>();
+  auto k = row_autocorr_kernel<R, N, W>;
+  hipError_t e = set_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(W * T), lds, s, a);
+  return hipGetLastError();
+}
+
+// Blocks row_autocorr_kernel uses for nlines lines of length N.
+inline int rowcorr_blocks(int64_t N, int64_t nlines) {
+  const int64_t W = x_lines_per_wg((int)N);
+  const int64_t need = (nlines + W - 1) / W;
+  return (int)(need < kRowCorrBlocks ? need : kRowCorrBlocks);
+}
+
 }  // namespace gpe
 
 #define GPE_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
@@ -178,6 +201,12 @@ static hipError_t launch_kwin(const KWinArgs& a, hipStream_t s) {
   hipError_t gpe_run_kwin_##SUFFIX(int64_t N, const gpe::KWinArgs& a, hipStream_t s) {                 \
     switch (N) {                                                                                       \
       GPE_SIZES(GPE_WCASE_##SUFFIX)                                                                    \
+    }                                                                                                  \
+    return hipErrorInvalidValue;                                                                       \
+  } \
+  hipError_t gpe_run_rowcorr_##SUFFIX(int64_t N, const gpe::RowCorrArgs& a, int nblk, hipStream_t s) {  \
+    switch (N) {                                                                                       \
+      GPE_SIZES(GPE_RCASE_##SUFFIX)                                                                    \
     }                                                                                                  \
     return hipErrorInvalidValue;                                                                       \
   }

@@ -17,7 +17,12 @@
 #define GPE_WCASE_f32(n) \
   case n: return gpe::launch_kwin<float, n>(a, s);
 
+#define GPE_RCASE_f32(n) \
+  case n: return gpe::launch_rowcorr<float, n>(a, nblk, s);
+
 #define GPE_SCASE_f32(n) \
   case n: return gpe::launch_sweep<float, n>(a, s);
 
 GPE_DEFINE_RUNNERS(float, f32)
+
+int gpe_rowcorr_blocks(int64_t N, int64_t nlines) { return gpe::rowcorr_blocks(N, nlines); }

@@ -18,6 +18,9 @@
 #define GPE_WCASE_f64(n) \
   case n: return gpe::launch_kwin<double, n>(a, s);
 
+#define GPE_RCASE_f64(n) \
+  case n: return gpe::launch_rowcorr<double, n>(a, nblk, s);
+
 #define GPE_SCASE_f64(n) \
   case n: return gpe::launch_sweep<double, n>(a, s);
 

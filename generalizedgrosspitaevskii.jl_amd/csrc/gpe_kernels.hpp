@@ -1110,6 +1110,110 @@ __global__ void __launch_bounds__(256) kspace_g1_kernel(const double* x1, const 
   g1[o + 1] = (si + ci) * scale;
 }
 
+// Density-density correlation along dimension 0 of an ensemble frame:
+// for every line (the other dimensions and the trajectories), with
+// n(x) = |u(x)|^2, the circular autocorrelation A(r) = sum_x n(x) n(x + r)
+// = IFFT(|FFT(n)|^2)(r) / N.  Each workgroup accumulates A(r) and sum_x n(x)
+// over the lines it owns (Neumaier-compensated FP64) and writes one partial
+// row of N + 1 values; row_autocorr_final_kernel sums the partial rows.
+struct RowCorrArgs {
+  const void* u;     // N x nlines complex values
+  int64_t nlines;
+  double* partial;   // gridDim.x rows of N + 1 values: A(r) sums, then sum n
+  const void* tw;
+  int tws;
+};
+
+template <typename R, int N, int W>
+__global__ void __launch_bounds__(W * (N / kE)) row_autocorr_kernel(RowCorrArgs p) {
+  constexpr int T = N / kE;
+  constexpr int LS = lds_line_stride(N);
+  constexpr bool LOCAL = T <= 64;
+  static_assert((size_t)W * N + (size_t)W * T <= 2ull * W * LS * sizeof(Cx<R>) / sizeof(double),
+                "the LDS of the transforms must hold the final reduction");
+  extern __shared__ unsigned char smem_raw[];
+  Cx<R>* lds = reinterpret_cast<Cx<R>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int l = tid / T, t = tid % T;
+  TwReg<R, N> tr;
+  load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(p.tw), p.tws);
+  double acc[kE], cmp[kE];
+#pragma unroll
+  for (int m = 0; m < kE; ++m) acc[m] = cmp[m] = 0.0;
+  double s1 = 0.0, c1 = 0.0;
+  const Cx<R>* u = reinterpret_cast<const Cx<R>*>(p.u);
+  for (int64_t base = (int64_t)blockIdx.x * W; base < p.nlines; base += (int64_t)gridDim.x * W) {
+    const int64_t line = base + l;
+    const bool valid = line < p.nlines;
+    const int64_t ln = valid ? line : p.nlines - 1;
+    Cx<R> v[kE];
+#pragma unroll
+    for (int m = 0; m < kE; ++m) {
+      const R n = abs2(u[ln * N + t + m * T]);
+      if (valid) neumaier_add(s1, c1, (double)n);
+      v[m] = mk<R>(n, (R)0);
+    }
+    fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), lds + l * LS, 0, W * LS, t, tr);
+#pragma unroll
+    for (int m = 0; m < kE; ++m) v[m] = mk<R>(abs2(v[m]), (R)0);
+    line_sync<LOCAL>();
+    fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), lds + l * LS, 0, W * LS, t, tr);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < kE; ++m) neumaier_add(acc[m], cmp[m], (double)v[m].re / N);
+    }
+    line_sync<LOCAL>();
+  }
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(smem_raw);
+#pragma unroll
+  for (int m = 0; m < kE; ++m) red[l * N + t + m * T] = acc[m] + cmp[m];
+  red[W * N + tid] = s1 + c1;
+  __syncthreads();
+  double* out = p.partial + (int64_t)blockIdx.x * (N + 1);
+  if (l == 0) {
+#pragma unroll
+    for (int m = 0; m < kE; ++m) {
+      double s = 0.0, c = 0.0;
+      for (int j = 0; j < W; ++j) neumaier_add(s, c, red[j * N + t + m * T]);
+      out[t + m * T] = s + c;
+    }
+  }
+  if (tid == 0) {
+    double s = 0.0, c = 0.0;
+    for (int j = 0; j < W * T; ++j) neumaier_add(s, c, red[W * N + j]);
+    out[N] = s + c;
+  }
+}
+
+// Sums the partial rows and forms, with npts = N * nlines and the mean
+// Wigner density nw = (sum n) / npts,
+//   g2r[r] = (sum A(r)) / npts - (1 + delta_r0) c/2 (2 nw - c/2),
+//   nphys  = nw - c/2
+// (the normal-ordered <n(x) n(x + r)> of examples/truncated_wigner.jl's
+// ordering correction, averaged over x, the other dimensions and the
+// trajectories).
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(256) row_autocorr_final_kernel(const double* partial, int nblk, int64_t n,
+                                                                 int64_t nlines, double comm, double* g2r,
+                                                                 double* nphys) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  double s = 0.0, c = 0.0, s1 = 0.0, c1 = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    neumaier_add(s, c, partial[(int64_t)b * (n + 1) + r]);
+    neumaier_add(s1, c1, partial[(int64_t)b * (n + 1) + n]);
+  }
+  const double npts = (double)n * (double)nlines;
+  const double nw = (s1 + c1) / npts;
+  if (r == n) {
+    nphys[0] = nw - comm / 2;
+    return;
+  }
+  const double d = r == 0 ? 2.0 : 1.0;
+  g2r[r] = (s + c) / npts - d * (comm / 2) * (2 * nw - comm / 2);
+}
+
 struct ResidualArgs {
   const void* a[kMaxM];  // u_{n+1} (real space)
   const void* b[kMaxM];  // u_n

@@ -81,3 +81,6 @@ hipError_t gpe_run_kabs2_f32(int64_t N, const gpe::KAbs2Args& a, hipStream_t s);
 hipError_t gpe_run_kabs2_f64(int64_t N, const gpe::KAbs2Args& a, hipStream_t s);
 hipError_t gpe_run_kwin_f32(int64_t N, const gpe::KWinArgs& a, hipStream_t s);
 hipError_t gpe_run_kwin_f64(int64_t N, const gpe::KWinArgs& a, hipStream_t s);
+hipError_t gpe_run_rowcorr_f32(int64_t N, const gpe::RowCorrArgs& a, int nblk, hipStream_t s);
+hipError_t gpe_run_rowcorr_f64(int64_t N, const gpe::RowCorrArgs& a, int nblk, hipStream_t s);
+int gpe_rowcorr_blocks(int64_t N, int64_t nlines);

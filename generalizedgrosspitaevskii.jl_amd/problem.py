@@ -714,6 +714,34 @@ class StrangSplittingIterator:
         sgn = np.where(idx % 2 == 0, 1.0, -1.0)
         return g[np.ix_(idx, idx)] * sgn[:, None] * sgn[None, :]
 
+    def density_correlation_x(self, frame, commutator):
+        """g2(r) along dimension 0 of one ensemble frame (1D or 2D, one
+        component, e.g. sol[c][..., f]), reduced on the device: returns
+        (G2, n_phys, g2) with G2[r] the normal-ordered <n(x) n(x + r)>
+        averaged over x, the other dimensions and the trajectories, n_phys the
+        mean physical density and g2 = G2 / n_phys^2 (the homogeneous-state
+        normalisation).  commutator: per grid point (1/dA in 2D, 1/dx in 1D);
+        the ordering correction is that of examples/truncated_wigner.jl:140-157."""
+        want = str(self.prob.u0[0].dtype).replace("torch.", "")
+        if _is_torch(frame) and frame.is_cuda:
+            f = frame if (_contig_colmajor(frame) and str(frame.dtype).replace("torch.", "") == want) \
+                else _to_colmajor(frame, want)
+            ptr = f.data_ptr()
+            _order_after_torch(f)
+        else:
+            f = np.asfortranarray(np.asarray(frame).astype(want))
+            ptr = f.ctypes.data
+        if int(np.prod(tuple(f.shape))) != int(np.prod(self.shape)):
+            raise ValueError(f"density_correlation_x: frame of shape {tuple(f.shape)} does not match {self.shape}")
+        n = self.spatial[0]
+        G2 = np.empty(n)
+        nphys = np.empty(1)
+        L.check(self._lib.gpe_density_g2_x(self.h, ptr, float(commutator), G2.ctypes.data, nphys.ctypes.data))
+        del f
+        with np.errstate(divide="ignore", invalid="ignore"):
+            g2 = G2 / nphys[0] ** 2
+        return G2, float(nphys[0]), g2
+
     def get_field(self):
         """The handle's current real-space field (host numpy arrays)."""
         out = tuple(np.empty(self.shape, dtype=str(x.dtype).replace("torch.", ""), order="F") for x in self.prob.u0)

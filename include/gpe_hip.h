@@ -217,6 +217,19 @@ int gpe_kspace_g2(gpe_handle* h, const void* frame, double commutator, double* n
  * k -> (k + N/2) mod N and the sign (-1)^k per index.  1D handles only. */
 int gpe_windowed_g1(gpe_handle* h, const void* frame, const double* w1, const double* w2, double* g1);
 
+/* Normal-ordered density-density correlation along dimension 0 of an
+ * ensemble frame (one component, `frame` = prod(n) x batch values in the
+ * field precision, host or device), averaged over x, the other dimensions and
+ * the trajectories, with the ordering correction of
+ * examples/truncated_wigner.jl:140-157 (delta = [r == 0]):
+ *   g2r[r] = < |u(x)|^2 |u(x + r)|^2 > - (1 + delta) c/2 (2 nw - c/2),
+ *   *nphys = nw - c/2,   nw = < |u(x)|^2 >,   r = 0 .. n[0]-1 (periodic),
+ * c = the commutator per grid point (1/dA, 1/dx).  g2(r) of a homogeneous
+ * state is g2r[r] / nphys^2.  FP64 with compensated sums; the
+ * autocorrelation of each line is formed through the FFT.  Used for the
+ * truncated-Wigner ensembles' g2(r) without moving frames to the host. */
+int gpe_density_g2_x(gpe_handle* h, const void* frame, double commutator, double* g2r, double* nphys);
+
 /* Copy the field after the last solve (real space) into u[c]. */
 int gpe_get_field(gpe_handle* h, void* const* u);
 

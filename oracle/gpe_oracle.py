@@ -22,6 +22,7 @@ It restates, in numpy complex128 arithmetic, what the reference computes for
 * FFT over the spatial dims        <- src/misc.jl:53-64 (FFTW plan_fft / inv)
 * ``kspace_g2`` (n(k), G2(k,k'))    <- examples/truncated_wigner.jl:110-165
 * ``windowed_correlation``, ``analytic_commutation`` <- test/windowed_ft.jl:31-59
+* ``density_g2_x`` (g2(r) along x)  <- examples/truncated_wigner.jl:140-157 (ordering correction)
 
 User closures follow the reference's calling conventions, evaluated
 vectorised over the grid instead of point by point:
@@ -535,3 +536,21 @@ def analytic_commutation(L, N, w1, w2):
     w = np.asarray(w1(rs), dtype=np.complex128) * np.conj(np.asarray(w2(rs), dtype=np.complex128))
     ph = np.exp(1j * (ks[:, None, None] - ks[None, :, None]) * rs[None, None, :])
     return (ph * w[None, None, :]).sum(axis=2) / (rs[-1] - rs[0]) / 2
+
+
+def density_g2_x(frames, commutator):
+    """Normal-ordered <n(x) n(x + r)> along axis 0 of an ensemble frame,
+    averaged over x and every other axis, with the ordering correction of
+    examples/truncated_wigner.jl:140-157 (f(x, y, delta, c) with
+    delta = [r == 0]), evaluated pointwise before averaging:
+        G2[r] = mean f(psi(x), psi(x + r), r == 0, c),  n_phys = mean |psi|^2 - c/2.
+    Returns (G2, n_phys)."""
+    a = np.abs(np.asarray(frames, dtype=np.complex128)) ** 2
+    c = float(commutator)
+    N = a.shape[0]
+    G2 = np.empty(N)
+    for r in range(N):
+        b = np.roll(a, -r, axis=0)  # b[x] = a[x + r]
+        d = 1.0 if r == 0 else 0.0
+        G2[r] = np.mean(a * b - (1 + d) * c / 2 * (a + b - c / 2))
+    return G2, float(a.mean() - c / 2)

@@ -117,3 +117,40 @@ def test_reference_windowed_ft_vacuum(ggp, orc):
             assert err <= 7e-2, (x0, w, err)
     it.close()
     print(f"windowed_ft vacuum: worst relative deviation {worst:.3f} (bar 7e-2)")
+
+
+@pytest.mark.parametrize("shape,dtype,tol", [((64, 32, 3), np.complex128, 1e-12), ((256, 40), np.complex64, 1e-5),
+                                             ((1024, 7), np.complex128, 1e-12), ((16, 16, 50), np.complex128, 1e-12)])
+def test_density_g2_x_matches_oracle(ggp, orc, shape, dtype, tol):
+    rng = np.random.default_rng(sum(shape))
+    frames = ((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * 0.5 + 0.3).astype(dtype)
+    nd = 2 if len(shape) == 3 else 1
+    u0 = (np.zeros(shape, dtype),)
+    prob = ggp.GrossPitaevskiiProblem(u0, (10.0,) * nd, dispersion=lambda k, p: sum(x ** 2 for x in k) / 2)
+    it = ggp.init(prob, ggp.StrangSplitting(), (0.0, 0.1), dt=0.05, nsaves=1)
+    c = 0.37
+    G2, nphys, g2 = it.density_correlation_x(frames, c)
+    it.close()
+    G2_o, n_o = orc.density_g2_x(frames.astype(np.complex128), c)
+    e = _rel(G2, G2_o)
+    print(f"density g2 {shape} {np.dtype(dtype).name}: {e:.2e}")
+    assert e <= tol and abs(nphys - n_o) <= tol * abs(n_o)
+    assert np.allclose(g2, G2 / nphys ** 2)
+
+
+def test_density_g2_x_twa_vacuum(ggp):
+    """A free 2D vacuum ensemble through solve: the normal-ordered density
+    correlation stays zero within its statistical error at every separation."""
+    N, B, L = 32, 2048, 8.0
+    dA = (L / N) ** 2
+    c = 1 / dA
+    rng = np.random.default_rng(13)
+    u = (rng.standard_normal((N, N, B)) + 1j * rng.standard_normal((N, N, B))) * np.sqrt(c / 4)
+    prob = ggp.GrossPitaevskiiProblem((u,), (L, L), dispersion=lambda k, p: (k[0] ** 2 + k[1] ** 2) / 2)
+    it = ggp.init(prob, ggp.StrangSplitting(), (0.0, 1.0), dt=0.05, nsaves=1)
+    ts, res = it.solve()
+    G2, nphys, g2 = it.density_correlation_x(res[0][..., -1], c)
+    it.close()
+    scale = (c / 2) ** 2 / np.sqrt(N * N * B)
+    assert abs(nphys) < 6 * (c / 2) / np.sqrt(N * N * B)
+    assert np.max(abs(G2)) < 30 * scale

@@ -396,3 +396,33 @@ def test_windowed_correlation_vacuum_matches_analytic(orc):
             corr = orc.windowed_correlation(u, f1(rs), f2(rs))
             an = orc.analytic_commutation(L, N, f1, f2)
             assert np.linalg.norm(corr - an) <= 7e-2 * max(np.linalg.norm(corr), np.linalg.norm(an))
+
+
+# ------------------------------------------------ density correlation g2(r)
+def test_density_g2_x_closed_forms(orc):
+    """A homogeneous coherent field and a static density wave
+    n(x) = 1 + eps cos(2 pi x / N) (the same in every trajectory) have closed
+    forms: <n(x) n(x + r)>_x = |a|^4, resp. 1 + eps^2/2 cos(2 pi r / N)."""
+    c = 0.2
+    alpha = 0.8 + 0.3j
+    G2, nphys = orc.density_g2_x(np.full((16, 4, 3), alpha), c)
+    A = abs(alpha) ** 2
+    want = A * A - (1 + (np.arange(16) == 0)) * c / 2 * (2 * A - c / 2)
+    assert np.allclose(G2, want, rtol=0, atol=1e-14) and abs(nphys - (A - c / 2)) < 1e-15
+    N, eps = 32, 0.4
+    x = np.arange(N)
+    u = np.sqrt(1 + eps * np.cos(2 * np.pi * x / N))[:, None, None] * np.ones((1, 8, 2))
+    G2, nphys = orc.density_g2_x(u, c)
+    r = np.arange(N)
+    want = 1 + eps ** 2 / 2 * np.cos(2 * np.pi * r / N) - (1 + (r == 0)) * c / 2 * (2 - c / 2)
+    assert np.allclose(G2, want, rtol=0, atol=1e-13) and abs(nphys - (1 - c / 2)) < 1e-14
+
+
+def test_density_g2_x_vacuum_is_uncorrelated(orc):
+    N, B, c = 16, 20000, 4.0
+    rng = np.random.default_rng(9)
+    u = (rng.standard_normal((N, B)) + 1j * rng.standard_normal((N, B))) * np.sqrt(c / 4)
+    G2, nphys = orc.density_g2_x(u, c)
+    scale = (c / 2) ** 2 / np.sqrt(N * B)
+    assert abs(nphys) < 6 * (c / 2) / np.sqrt(N * B)
+    assert np.max(abs(G2)) < 30 * scale
