@@ -80,6 +80,7 @@ SIGNATURES = {
     "gpe_set_dispersion": (ctypes.c_int, [P, I32, P]),
     "gpe_set_potential": (ctypes.c_int, [P, I32, P]),
     "gpe_set_nonlinearity": (ctypes.c_int, [P, P, P]),
+    "gpe_set_nonlinearity_matrix": (ctypes.c_int, [P, P, P]),
     "gpe_set_pump": (ctypes.c_int, [P, I32, P, P, I64]),
     "gpe_set_noise": (ctypes.c_int, [P, I32, P, ctypes.c_uint64, ctypes.c_uint64]),
     "gpe_set_observables": (ctypes.c_int, [P, P, P]),

@@ -85,6 +85,10 @@ struct gpe_handle {
   int g_coupled = 0;
   double g[kMaxM][kMaxM][2] = {};
   double c[kMaxM][2] = {};
+  // 2x2 matrix nonlinearity G_ab = gm_c[a][b] + sum_j gm_g[a][b][j] |u_j|^2
+  bool gmat = false;
+  double gm_g[2][2][2][2] = {};
+  double gm_c[2][2][2] = {};
   // pump
   int pump_kind = GPE_PUMP_NONE;
   void* pump_a[kMaxM] = {nullptr, nullptr};  // active profile per component (nullptr: F_c = 0)
@@ -459,6 +463,27 @@ extern "C" int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double
   for (int i = 0; i < M; ++i)
     for (int j = 0; j < M; ++j)
       if (i != j && (h->g[i][j][0] != 0.0 || h->g[i][j][1] != 0.0)) h->g_coupled = 1;
+  h->gmat = false;
+  return GPE_OK;
+}
+
+extern "C" int gpe_set_nonlinearity_matrix(gpe_handle* h, const double* g, const double* c) {
+  if (!h || !g || !c) return fail(GPE_ERR_INVALID, "gpe_set_nonlinearity_matrix: null argument");
+  if (h->M != 2) return fail(GPE_ERR_UNSUPPORTED, "gpe_set_nonlinearity_matrix: two components only (M=%d)", h->M);
+  if (h->stream) HIPCHK(hipStreamSynchronize(h->stream));
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      for (int q = 0; q < 2; ++q) h->gm_c[a][b][q] = c[2 * (a + 2 * b) + q];
+      for (int j = 0; j < 2; ++j)
+        for (int q = 0; q < 2; ++q) h->gm_g[a][b][j][q] = g[2 * ((a + 2 * b) + 4 * j) + q];
+    }
+  // the vector-form G of the fused x-pass is off while the matrix form is set
+  h->gmask = 0;
+  h->g_cplx = 0;
+  h->g_coupled = 0;
+  memset(h->g, 0, sizeof h->g);
+  memset(h->c, 0, sizeof h->c);
+  h->gmat = true;
   return GPE_OK;
 }
 
@@ -701,7 +726,7 @@ static Step1DArgs s1_args(gpe_handle* h, double dt) {
 static uint64_t plan_key_of(const gpe_handle* h) {
   return 1 + (uint64_t)(h->noise_kind != GPE_NOISE_NONE) + 2 * (uint64_t)h->g_coupled + 4 * (uint64_t)h->g_cplx +
          8 * (uint64_t)h->gmask + 32 * (uint64_t)h->disp_form + 256 * (uint64_t)h->pot_form +
-         4096 * (uint64_t)h->pump_kind;
+         4096 * (uint64_t)h->pump_kind + 65536 * (uint64_t)h->gmat;
 }
 
 // Mean device time of one call of fn (the launches it issues), from HIP
@@ -955,7 +980,7 @@ struct GraphExecGuard {
 constexpr int64_t kGraphMinSaves = 8;
 
 static bool graph_eligible(const gpe_handle* h, int64_t sps) {
-  return h->ndim == 2 && h->noise_kind == GPE_NOISE_NONE && h->env.empty() && sps >= 2;
+  return h->ndim == 2 && h->noise_kind == GPE_NOISE_NONE && h->env.empty() && sps >= 2 && !h->gmat;
 }
 
 static bool use_step_graph(const gpe_handle* h, int64_t sps, int64_t nsaves) {
@@ -992,6 +1017,9 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     }
     HIPCHK(hipMemcpyAsync(h->d_env, h->env.data(), need * sizeof(double), hipMemcpyHostToDevice, h->stream));
   }
+  if (h->gmat && (h->ndim != 2 || h->noise_kind != GPE_NOISE_NONE || h->pot_form != GPE_FORM_NONE))
+    return fail(GPE_ERR_UNSUPPORTED,
+                "gpe_solve: a matrix nonlinearity runs on 2D grids without noise or potential");
   const bool graph = use_step_graph(h, sps, nsaves);
   if (h->schedule == GPE_SCHEDULE_GRAPH && !graph)
     return fail(GPE_ERR_UNSUPPORTED,
@@ -1104,6 +1132,59 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
       rc = flush_frame(f);
       if (rc) return rc;
       if (ts) ts[n + (save_start ? 1 : 0)] = t;
+    }
+  } else if (h->gmat) {
+    // matrix G: the x-passes only transform; gmat_half_kernel applies the
+    // half-steps (and writes the frames) in real space between them
+    XPassArgs xa = x_args(h);
+    const YPassArgs ya = y_args(h);
+    GmatArgs ga;
+    memset(&ga, 0, sizeof ga);
+    for (int c = 0; c < 2; ++c) {
+      ga.u[c] = h->u[c];
+      ga.pump_a[c] = h->pump_a[c];
+      ga.pump_b[c] = h->pump_b[c];
+    }
+    ga.npts = h->npts;
+    ga.n0 = h->n0;
+    ga.nsp = h->nsp;
+    ga.pump_kind = h->pump_kind;
+    memcpy(ga.g, h->gm_g, sizeof ga.g);
+    memcpy(ga.c, h->gm_c, sizeof ga.c);
+    const unsigned gblocks = (unsigned)((h->npts + 255) / 256);
+    auto run_gm = [&]() -> hipError_t {
+      if (h->f64)
+        hipLaunchKernelGGL(gmat_half_kernel<double>, dim3(gblocks), dim3(256), 0, h->stream, ga);
+      else
+        hipLaunchKernelGGL(gmat_half_kernel<float>, dim3(gblocks), dim3(256), 0, h->stream, ga);
+      return hipGetLastError();
+    };
+    ga.flags = kStart;
+    ga.h_start = make_half(h, rdt, 1, 0, call0);
+    HIPCHK(run_gm());
+    xa.flags = kOutK;
+    HIPCHK(run_x(h, xa));
+    for (int64_t s = 1; s <= S; ++s) {
+      t += rdt;
+      HIPCHK(run_y(h, ya));
+      xa.flags = kInK;
+      HIPCHK(run_x(h, xa));
+      const bool save = (s % sps) == 0;
+      const int64_t f = s / sps - 1 + (save_start ? 1 : 0);
+      for (int c = 0; c < kMaxM; ++c) ga.frame[c] = save ? frame_ptr(c, f) : nullptr;
+      ga.flags = kEnd | (s < S ? kStart : 0);
+      ga.h_end = make_half(h, rdt, s, 1, call0);
+      if (s < S) ga.h_start = make_half(h, rdt, s + 1, 0, call0);
+      HIPCHK(run_gm());
+      if (s < S) {
+        xa.flags = kOutK;
+        HIPCHK(run_x(h, xa));
+      }
+      if (save) {
+        rc = flush_frame(f);
+        if (rc) return rc;
+        if (ts) ts[s / sps - (save_start ? 0 : 1)] = t;
+      }
     }
   } else if (h->ndim == 2) {
     XPassArgs xa = x_args(h);
@@ -1313,6 +1394,7 @@ extern "C" int gpe_last_solve_ms(gpe_handle* h, float* ms) {
 static int advance_impl(gpe_handle* h, int64_t nsteps, bool per_kernel, double* ms_x, double* ms_y) {
   if (!h->configured) return fail(GPE_ERR_STATE, "gpe_advance: no previous gpe_solve");
   if (nsteps < 1) return fail(GPE_ERR_INVALID, "gpe_advance: nsteps=%lld", (long long)nsteps);
+  if (h->gmat) return fail(GPE_ERR_UNSUPPORTED, "gpe_advance: matrix nonlinearity not supported");
   HIPCHK(hipSetDevice(h->d.device));
   const std::vector<double> saved_env = h->env;
   if (!h->env.empty()) {
@@ -1777,6 +1859,7 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
     return fail(GPE_ERR_INVALID, "gpe_steady_state: dt=%g tol=%g check_every=%lld max_steps=%lld", dt, tol,
                 (long long)check_every, (long long)max_steps);
   if (!h->env.empty()) return fail(GPE_ERR_STATE, "gpe_steady_state: the pump envelope must be static");
+  if (h->gmat) return fail(GPE_ERR_UNSUPPORTED, "gpe_steady_state: matrix nonlinearity not supported");
   HIPCHK(hipSetDevice(h->d.device));
   int rc = prepare_tables(h, dt);
   if (rc) return rc;

@@ -739,6 +739,80 @@ __global__ void ed_table_kernel(EdArgs a) {
 }
 
 // exp(-i tau V) planes (potential, get_exponential with dt/2), per component.
+// --------------------------------------- matrix nonlinearity half-steps
+// The real-space half-step of src/kernels.jl:37-54 when nonlinearity(u) is a
+// 2x2 SMatrix: cis(-delta G(u)) is then a per-point matrix exponential
+// (LinearAlgebra's cis of a matrix) applied to the component vector,
+//   u <- exp(-i delta G(u)) (u + delta/2 F_now) + delta/2 F_next,
+// with G_ab(u) = c_ab + sum_j g_abj |u_j|^2 from the pre-update fields, in
+// FP64.  A 2D step runs it between an x-pass that only inverse-transforms and
+// one that only transforms.
+struct GmatArgs {
+  void* u[kMaxM];
+  void* frame[kMaxM];   // real-space copy after the end half-step, or null
+  int64_t npts;         // points per component, all trajectories
+  int64_t n0, nsp;      // dim-0 extent, points per grid
+  int flags;            // kEnd | kStart
+  HalfStep h_end, h_start;
+  double g[2][2][2][2]; // g[a][b][j] complex
+  double c[2][2][2];    // c[a][b] complex
+  int pump_kind;        // 0 none, 1 table, 2 separable
+  const void* pump_a[kMaxM];
+  const void* pump_b[kMaxM];
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<double> gmat_pump(const GmatArgs& a, int comp, int64_t p) {
+  const void* pa = a.pump_a[comp];
+  if (a.pump_kind == 0 || pa == nullptr) return mk<double>(0.0, 0.0);
+  const int64_t sp = p % a.nsp;
+  Cx<R> v;
+  if (a.pump_kind == 1) {
+    v = ldc<R>(pa, sp);
+  } else {
+    const Cx<R> pb = a.pump_b[comp] ? ldc<R>(a.pump_b[comp], sp / a.n0) : mk<R>((R)1, (R)0);
+    v = ldc<R>(pa, sp % a.n0) * pb;
+  }
+  return mk<double>((double)v.re, (double)v.im);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) gmat_half_kernel(GmatArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.npts) return;
+  Cx<R>* u0 = reinterpret_cast<Cx<R>*>(a.u[0]);
+  Cx<R>* u1 = reinterpret_cast<Cx<R>*>(a.u[1]);
+  Cx<double> v[2] = {mk<double>((double)u0[p].re, (double)u0[p].im), mk<double>((double)u1[p].re, (double)u1[p].im)};
+  const Cx<double> F[2] = {gmat_pump<R>(a, 0, p), gmat_pump<R>(a, 1, p)};
+  auto half = [&](const HalfStep& h) {
+    const double a2[2] = {abs2(v[0]), abs2(v[1])};
+    Cx<double> A[4];  // -i delta G, column-major
+    for (int col = 0; col < 2; ++col)
+      for (int row = 0; row < 2; ++row) {
+        double gr = a.c[row][col][0], gi = a.c[row][col][1];
+        for (int j = 0; j < 2; ++j) {
+          gr += a.g[row][col][j][0] * a2[j];
+          gi += a.g[row][col][j][1] * a2[j];
+        }
+        A[row + 2 * col] = mk<double>(h.delta * gi, -h.delta * gr);
+      }
+    Cx<double> E[4];
+    expm2x2_d(A, E);
+    const Cx<double> fn = mk<double>(h.fnow[0], h.fnow[1]), fx = mk<double>(h.fnext[0], h.fnext[1]);
+    const Cx<double> w0 = v[0] + F[0] * fn, w1 = v[1] + F[1] * fn;
+    v[0] = E[0] * w0 + E[2] * w1 + F[0] * fx;
+    v[1] = E[1] * w0 + E[3] * w1 + F[1] * fx;
+  };
+  if (a.flags & kEnd) half(a.h_end);
+  if (a.frame[0]) {
+    reinterpret_cast<Cx<R>*>(a.frame[0])[p] = mk<R>((R)v[0].re, (R)v[0].im);
+    reinterpret_cast<Cx<R>*>(a.frame[1])[p] = mk<R>((R)v[1].re, (R)v[1].im);
+  }
+  if (a.flags & kStart) half(a.h_start);
+  u0[p] = mk<R>((R)v[0].re, (R)v[0].im);
+  u1[p] = mk<R>((R)v[1].re, (R)v[1].im);
+}
+
 struct EvArgs {
   const double* V;  // complex table
   void* out[kMaxM];

@@ -217,6 +217,10 @@ def _probe_fields(M, K, seed):
     return tuple(mags[c] * np.exp(1j * ph[c]) for c in range(M))
 
 
+class MatrixNonlinearity(NotImplementedError):
+    """nonlinearity(u, param) returns a 2x2 SMatrix with off-diagonal terms."""
+
+
 def fit_nonlinearity(f, M, param):
     """Reduce nonlinearity(u, param) to G_c = c_c + sum_j g_cj |u_j|^2.
 
@@ -235,8 +239,7 @@ def fit_nonlinearity(f, M, param):
             raise ValueError("nonlinearity: SMatrix size mismatch")
         off = max((np.max(np.abs(arr[i, j])) for i in range(M) for j in range(M) if i != j), default=0.0)
         if off != 0.0:
-            raise NotImplementedError("nonlinearity: non-diagonal SMatrix G(u) needs a per-point matrix "
-                                      "exponential, not supported by libgpe_hip")
+            raise MatrixNonlinearity("nonlinearity: non-diagonal SMatrix G(u) (use fit_nonlinearity_matrix)")
         G = np.stack([arr[i, i] for i in range(M)])
     A = np.column_stack([np.ones(K)] + [np.abs(u[j]) ** 2 for j in range(M)])
     g = np.zeros((M, M), dtype=np.complex128)
@@ -261,6 +264,40 @@ def fit_nonlinearity(f, M, param):
         arr2 if kind2 == "V" else np.stack([arr2[i, i] for i in range(M)]))
     pred = c[:, None] + g @ np.stack([np.abs(x) ** 2 for x in u2])
     if np.max(np.abs(pred - G2)) > 1e-11 * (1.0 + np.max(np.abs(G2))):
+        raise NotImplementedError("nonlinearity: probe verification failed")
+    return g, c
+
+
+def fit_nonlinearity_matrix(f, M, param):
+    """Reduce an SMatrix nonlinearity(u, param) (src/kernels.jl:44, cis of a
+    matrix) to G_ab = c_ab + sum_j g_abj |u_j|^2 for two components.
+
+    Returns (g[a][b][j], c[a][b]) complex or raises NotImplementedError."""
+    if M != 2:
+        raise NotImplementedError("nonlinearity: matrix G(u) is supported for two components")
+    K = 64
+    coefs = []
+    for seed in (0x6E4E, 0x6E4F):
+        u = _probe_fields(M, K, seed)
+        kind, arr = _form(f(u, param), (K,))
+        if kind != "M" or arr.shape[:2] != (2, 2):
+            raise NotImplementedError("nonlinearity: expected a 2x2 SMatrix")
+        A = np.column_stack([np.ones(K)] + [np.abs(u[j]) ** 2 for j in range(M)]).astype(np.complex128)
+        g = np.zeros((2, 2, 2), dtype=np.complex128)
+        c = np.zeros((2, 2), dtype=np.complex128)
+        for a in range(2):
+            for b in range(2):
+                coef, *_ = np.linalg.lstsq(A, arr[a, b], rcond=None)
+                if np.max(np.abs(A @ coef - arr[a, b])) > 1e-11 * (1.0 + np.max(np.abs(arr[a, b]))):
+                    raise NotImplementedError("nonlinearity: matrix entry is not c + sum_j g_j |u_j|^2")
+                c[a, b] = coef[0]
+                g[a, b] = coef[1:]
+        tiny = 1e-14 * (1.0 + np.max(np.abs(g)) + np.max(np.abs(c)))
+        g[np.abs(g) < tiny] = 0
+        c[np.abs(c) < tiny] = 0
+        coefs.append((g, c))
+    (g, c), (g2, c2) = coefs
+    if np.max(np.abs(g - g2)) > 1e-10 * (1 + np.max(np.abs(g))) or np.max(np.abs(c - c2)) > 1e-10 * (1 + np.max(np.abs(c))):
         raise NotImplementedError("nonlinearity: probe verification failed")
     return g, c
 
@@ -571,12 +608,24 @@ class StrangSplittingIterator:
             self._keep.append(tab)
             L.check(lib.gpe_set_potential(h, form, tab.ctypes.data))
         if "nonlinearity" in which and prob.nonlinearity is not None:
-            g, c = fit_nonlinearity(prob.nonlinearity, M, prob.param)
-            # column-major g[i + M j]
-            gbuf = np.ascontiguousarray(g.T.ravel()).view(np.float64)
-            cbuf = np.ascontiguousarray(c).view(np.float64)
-            self._keep += [gbuf, cbuf]
-            L.check(lib.gpe_set_nonlinearity(h, gbuf.ctypes.data, cbuf.ctypes.data))
+            try:
+                g, c = fit_nonlinearity(prob.nonlinearity, M, prob.param)
+            except MatrixNonlinearity:
+                gm, cm = fit_nonlinearity_matrix(prob.nonlinearity, M, prob.param)
+                # layout: g[(a + 2b) + 4j], c[a + 2b]
+                gbuf = np.ascontiguousarray([gm[a, b, j] for j in range(2) for b in range(2) for a in range(2)],
+                                            dtype=np.complex128).view(np.float64)
+                cbuf = np.ascontiguousarray([cm[a, b] for b in range(2) for a in range(2)],
+                                            dtype=np.complex128).view(np.float64)
+                self._keep += [gbuf, cbuf]
+                L.check(lib.gpe_set_nonlinearity_matrix(h, gbuf.ctypes.data, cbuf.ctypes.data))
+                g = None
+            if g is not None:
+                # column-major g[i + M j]
+                gbuf = np.ascontiguousarray(g.T.ravel()).view(np.float64)
+                cbuf = np.ascontiguousarray(c).view(np.float64)
+                self._keep += [gbuf, cbuf]
+                L.check(lib.gpe_set_nonlinearity(h, gbuf.ctypes.data, cbuf.ctypes.data))
         if "pump" in which and prob.pump is not None:
             times = pump_times(self.tspan, self.dt_in, self.nsaves)
             kind, prof, env = pump_descriptor(prob, rax, spatial, times)

@@ -95,6 +95,17 @@ int gpe_set_potential(gpe_handle* h, int32_t form, const double* V);
  * (src/kernels.jl:44).  g == NULL and c == NULL clears it. */
 int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double* c);
 
+/* Matrix nonlinearity (two components): nonlinearity(u, param) returning a
+ * 2x2 SMatrix, which the real-space half-step applies as the matrix
+ * exponential cis(-dt/2 G(u)) to the component vector (src/kernels.jl:23,44;
+ * LinearAlgebra's cis of a matrix), in the descriptor form
+ *     G_ab(u) = c[a + 2b] + sum_j g[(a + 2b) + 4j] |u_j|^2     (complex FP64).
+ * Replaces the vector form until gpe_set_nonlinearity is called again.
+ * 2D grids without noise or potential (GPE_ERR_UNSUPPORTED from gpe_solve
+ * otherwise); the half-steps run in a per-point kernel between transform-only
+ * row passes. */
+int gpe_set_nonlinearity_matrix(gpe_handle* h, const double* g, const double* c);
+
 /* Pump F(r,t) = profile(r) * envelope(t) (src/misc.jl:22-42).
  *   GPE_PUMP_TABLE:     profile = M blocks of prod(n) complex values.
  *   GPE_PUMP_SEPARABLE: profile = M blocks of (n[0] + ... + n[ndim-1])

@@ -502,3 +502,46 @@ def test_auto_planner_measures_long_runs_once_per_process(ggp):
     res = it.solve()
     it.close()
     assert np.array_equal(short[1][0], res[1][0])
+
+
+# ------------------------------------------------ matrix nonlinearity (SMatrix G)
+def rabi_terms(L, W=0.4, gamma=0.05):
+    """Two coupled condensates: G = [[g n0 + g12 n1 - i gamma, W], [W, g n1 + g12 n0]]
+    (an SMatrix nonlinearity, src/kernels.jl:44), a Gaussian pump on
+    component 0 and the free dispersion."""
+    def nonlinearity(u, p):
+        n0, n1 = abs(u[0]) ** 2, abs(u[1]) ** 2
+        return [[p["g"] * n0 + p["g12"] * n1 - 1j * p["gam"], p["W"] + 0 * n0],
+                [p["W"] + 0 * n0, p["g"] * n1 + p["g12"] * n0]]
+
+    def pump(r, p, t):
+        f = 0.8 * np.exp(-sum((x - p["L"] / 2) ** 2 for x in r) / (p["L"] / 5) ** 2)
+        return [f, 0 * f]
+
+    return dict(dispersion=disp_kin, nonlinearity=nonlinearity, pump=pump,
+                param=dict(g=1.0, g12=0.6, W=W, gam=gamma, L=L))
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, FP64_TOL), (np.complex64, FP32_TOL)])
+@pytest.mark.parametrize("save_start", [True, False])
+def test_matrix_nonlinearity_matches_oracle(ggp, orc, dtype, tol, save_start):
+    N, L = 64, 12.0
+    rng = np.random.default_rng(17)
+    u0 = tuple(0.3 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) for _ in range(2))
+    out = run_both(ggp, orc, u0, (L, L), (0, 1.0), 0.02, 5, dtype=dtype, save_start=save_start, **rabi_terms(L))
+    err = check(*out, tol)
+    print(f"matrix G {np.dtype(dtype).name}: rel L2 {err:.3e}")
+
+
+def test_matrix_nonlinearity_limits(ggp):
+    """Noise or a 1D grid with a matrix G is refused, not approximated."""
+    L = 12.0
+    u0 = (np.zeros((32, 32), complex), np.zeros((32, 32), complex))
+    terms = rabi_terms(L)
+    noisy = ggp.GrossPitaevskiiProblem(u0, (L, L), position_noise_func=lambda u, r, p: [0.1, 0.1],
+                                       noise_prototype=tuple(np.empty((32, 32), complex) for _ in range(2)), **terms)
+    with pytest.raises(ggp.GpeUnsupported):
+        ggp.solve(noisy, ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1, rng=ggp.PhiloxRNG(3))
+    one_d = ggp.GrossPitaevskiiProblem((np.zeros(64, complex), np.zeros(64, complex)), (L,), **rabi_terms(L))
+    with pytest.raises(ggp.GpeUnsupported):
+        ggp.solve(one_d, ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1)

@@ -42,6 +42,26 @@ def test_nonlinearity_fit(ggp):
         P.fit_nonlinearity(lambda u, p: [[0 * u[0], 1 + 0 * u[0]], [1 + 0 * u[0], 0 * u[0]]], 2, None)
 
 
+def test_fit_nonlinearity_matrix(ggp):
+    """An off-diagonal SMatrix G(u) (Rabi coupling plus cross-phase terms)
+    reduces to G_ab = c_ab + sum_j g_abj |u_j|^2."""
+    from gpe_hip_amd import problem as P
+
+    def G(u, p):
+        n0, n1 = abs(u[0]) ** 2, abs(u[1]) ** 2
+        return [[p["g"] * n0 + p["g12"] * n1 - 0.1j, p["W"] + 0 * n0], [p["W"] + 0.2 * n1, p["g"] * n1 + p["g12"] * n0]]
+
+    g, c = P.fit_nonlinearity_matrix(G, 2, dict(g=1.5, g12=0.5, W=0.3))
+    want_c = np.array([[-0.1j, 0.3], [0.3, 0]])
+    want_g = np.zeros((2, 2, 2), complex)
+    want_g[0, 0] = [1.5, 0.5]
+    want_g[1, 1] = [0.5, 1.5]
+    want_g[1, 0] = [0, 0.2]
+    assert np.allclose(c, want_c, atol=1e-13) and np.allclose(g, want_g, atol=1e-13)
+    with pytest.raises(NotImplementedError):
+        P.fit_nonlinearity_matrix(lambda u, p: [[abs(u[0]) ** 4, 0 * u[0]], [1 + 0 * u[0], 0 * u[0]]], 2, None)
+
+
 def test_noise_eta(ggp):
     from gpe_hip_amd import problem as P
 

@@ -426,3 +426,19 @@ def test_density_g2_x_vacuum_is_uncorrelated(orc):
     scale = (c / 2) ** 2 / np.sqrt(N * B)
     assert abs(nphys) < 6 * (c / 2) / np.sqrt(N * B)
     assert np.max(abs(G2)) < 30 * scale
+
+
+def test_matrix_nonlinearity_constant_rabi_closed_form(orc):
+    """A constant 2x2 SMatrix G (Rabi coupling, complex detuning) and no
+    dispersion: every half-step is u <- exp(-i dt/2 G) u, so after time t the
+    field is exp(-i t G) u0 exactly (scipy's expm)."""
+    G = np.array([[0.3 - 0.05j, 0.7], [0.7, -0.2]])
+    rng = np.random.default_rng(4)
+    u0 = tuple(rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8)) for _ in range(2))
+    prob = orc.Problem(u0, (4.0, 4.0), nonlinearity=lambda u, p: [[G[0, 0] + 0 * u[0], G[0, 1] + 0 * u[0]],
+                                                                   [G[1, 0] + 0 * u[0], G[1, 1] + 0 * u[0]]])
+    ts, res, _ = orc.solve(prob, (0, 1.0), 0.05, 2)
+    E = scipy.linalg.expm(-1j * ts[-1] * G)
+    want = np.einsum("ab,bxy->axy", E, np.stack(u0))
+    got = np.stack([res[c][..., -1] for c in range(2)])
+    assert np.max(abs(got - want)) < 1e-12 * np.max(abs(want))
