@@ -1017,9 +1017,8 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     }
     HIPCHK(hipMemcpyAsync(h->d_env, h->env.data(), need * sizeof(double), hipMemcpyHostToDevice, h->stream));
   }
-  if (h->gmat && (h->ndim != 2 || h->noise_kind != GPE_NOISE_NONE || h->pot_form != GPE_FORM_NONE))
-    return fail(GPE_ERR_UNSUPPORTED,
-                "gpe_solve: a matrix nonlinearity runs on 2D grids without noise or potential");
+  if (h->gmat && (h->noise_kind != GPE_NOISE_NONE || h->pot_form != GPE_FORM_NONE))
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_solve: a matrix nonlinearity runs without noise or potential");
   const bool graph = use_step_graph(h, sps, nsaves);
   if (h->schedule == GPE_SCHEDULE_GRAPH && !graph)
     return fail(GPE_ERR_UNSUPPORTED,
@@ -1137,7 +1136,8 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     // matrix G: the x-passes only transform; gmat_half_kernel applies the
     // half-steps (and writes the frames) in real space between them
     XPassArgs xa = x_args(h);
-    const YPassArgs ya = y_args(h);
+    // (1D handles have no column pass: y_args would divide by n1 / 8 = 0)
+    const YPassArgs ya = h->ndim == 2 ? y_args(h) : YPassArgs{};
     GmatArgs ga;
     memset(&ga, 0, sizeof ga);
     for (int c = 0; c < 2; ++c) {
@@ -1159,16 +1159,29 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
         hipLaunchKernelGGL(gmat_half_kernel<float>, dim3(gblocks), dim3(256), 0, h->stream, ga);
       return hipGetLastError();
     };
+    // 1D: the whole-step kernel with every real-space term off does the
+    // k-space part of one step (its half-steps are then the identity)
+    Step1DArgs ka = s1_args(h, rdt);
+    memset(&ka.pt, 0, sizeof ka.pt);
+    ka.env = nullptr;
+    ka.nsteps = 1;
+    for (int c = 0; c < kMaxM; ++c) ka.frame[c] = nullptr;
     ga.flags = kStart;
     ga.h_start = make_half(h, rdt, 1, 0, call0);
     HIPCHK(run_gm());
-    xa.flags = kOutK;
-    HIPCHK(run_x(h, xa));
+    if (h->ndim == 2) {
+      xa.flags = kOutK;
+      HIPCHK(run_x(h, xa));
+    }
     for (int64_t s = 1; s <= S; ++s) {
       t += rdt;
-      HIPCHK(run_y(h, ya));
-      xa.flags = kInK;
-      HIPCHK(run_x(h, xa));
+      if (h->ndim == 2) {
+        HIPCHK(run_y(h, ya));
+        xa.flags = kInK;
+        HIPCHK(run_x(h, xa));
+      } else {
+        HIPCHK(run_1d(h, ka));
+      }
       const bool save = (s % sps) == 0;
       const int64_t f = s / sps - 1 + (save_start ? 1 : 0);
       for (int c = 0; c < kMaxM; ++c) ga.frame[c] = save ? frame_ptr(c, f) : nullptr;
@@ -1176,7 +1189,7 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
       ga.h_end = make_half(h, rdt, s, 1, call0);
       if (s < S) ga.h_start = make_half(h, rdt, s + 1, 0, call0);
       HIPCHK(run_gm());
-      if (s < S) {
+      if (s < S && h->ndim == 2) {
         xa.flags = kOutK;
         HIPCHK(run_x(h, xa));
       }

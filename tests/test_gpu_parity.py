@@ -533,8 +533,18 @@ def test_matrix_nonlinearity_matches_oracle(ggp, orc, dtype, tol, save_start):
     print(f"matrix G {np.dtype(dtype).name}: rel L2 {err:.3e}")
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, FP64_TOL), (np.complex64, FP32_TOL)])
+def test_matrix_nonlinearity_1d_batched_matches_oracle(ggp, orc, dtype, tol):
+    N, B, L = 128, 3, 12.0
+    rng = np.random.default_rng(23)
+    u0 = tuple(0.3 * (rng.standard_normal((N, B)) + 1j * rng.standard_normal((N, B))) for _ in range(2))
+    out = run_both(ggp, orc, u0, (L,), (0, 1.0), 0.02, 5, dtype=dtype, **rabi_terms(L))
+    err = check(*out, tol)
+    print(f"matrix G 1D {np.dtype(dtype).name}: rel L2 {err:.3e}")
+
+
 def test_matrix_nonlinearity_limits(ggp):
-    """Noise or a 1D grid with a matrix G is refused, not approximated."""
+    """Noise with a matrix G is refused, not approximated."""
     L = 12.0
     u0 = (np.zeros((32, 32), complex), np.zeros((32, 32), complex))
     terms = rabi_terms(L)
@@ -542,6 +552,3 @@ def test_matrix_nonlinearity_limits(ggp):
                                        noise_prototype=tuple(np.empty((32, 32), complex) for _ in range(2)), **terms)
     with pytest.raises(ggp.GpeUnsupported):
         ggp.solve(noisy, ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1, rng=ggp.PhiloxRNG(3))
-    one_d = ggp.GrossPitaevskiiProblem((np.zeros(64, complex), np.zeros(64, complex)), (L,), **rabi_terms(L))
-    with pytest.raises(ggp.GpeUnsupported):
-        ggp.solve(one_d, ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1)
