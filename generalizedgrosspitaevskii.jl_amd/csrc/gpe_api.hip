@@ -1017,8 +1017,8 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     }
     HIPCHK(hipMemcpyAsync(h->d_env, h->env.data(), need * sizeof(double), hipMemcpyHostToDevice, h->stream));
   }
-  if (h->gmat && (h->noise_kind != GPE_NOISE_NONE || h->pot_form != GPE_FORM_NONE))
-    return fail(GPE_ERR_UNSUPPORTED, "gpe_solve: a matrix nonlinearity runs without noise or potential");
+  if (h->gmat && h->pot_form != GPE_FORM_NONE)
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_solve: a matrix nonlinearity runs without a potential");
   const bool graph = use_step_graph(h, sps, nsaves);
   if (h->schedule == GPE_SCHEDULE_GRAPH && !graph)
     return fail(GPE_ERR_UNSUPPORTED,
@@ -1149,6 +1149,10 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     ga.n0 = h->n0;
     ga.nsp = h->nsp;
     ga.pump_kind = h->pump_kind;
+    ga.noise_kind = h->noise_kind;
+    for (int c = 0; c < 2; ++c) ga.eta[c] = h->eta[c];
+    ga.seed_lo = (uint32_t)h->seed;
+    ga.seed_hi = (uint32_t)(h->seed >> 32);
     memcpy(ga.g, h->gm_g, sizeof ga.g);
     memcpy(ga.c, h->gm_c, sizeof ga.c);
     const unsigned gblocks = (unsigned)((h->npts + 255) / 256);

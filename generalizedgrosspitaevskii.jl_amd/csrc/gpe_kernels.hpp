@@ -759,7 +759,31 @@ struct GmatArgs {
   int pump_kind;        // 0 none, 1 table, 2 separable
   const void* pump_a[kMaxM];
   const void* pump_b[kMaxM];
+  int noise_kind;       // 0 none, 1 complex, 2 real (as PointTerms)
+  double eta[kMaxM];
+  uint32_t seed_lo, seed_hi;
 };
+
+// The Wiener increment of component c at flat point p of a half-step, drawn
+// exactly as noise_terms does for the fused kernels: line = p / n0, the pair
+// index P = line * n0/2 + (x mod n0/2) and words (x, y) for x < n0/2, (z, w)
+// for the other half of the line.
+template <typename R>
+__device__ __forceinline__ Cx<double> gmat_noise(const GmatArgs& a, const HalfStep& h, int c, int64_t p) {
+  const int64_t half = a.n0 / 2;
+  const int64_t x = p % a.n0, line = p / a.n0;
+  const uint64_t P = (uint64_t)(line * half + (x % half));
+  U4 ctr;
+  ctr.x = (uint32_t)P;
+  ctr.y = (uint32_t)c + 16u * (uint32_t)(P >> 32);
+  ctr.z = (uint32_t)h.call;
+  ctr.w = (uint32_t)(h.call >> 32);
+  const U4 w = philox4x32_10(ctr, a.seed_lo, a.seed_hi);
+  Cx<R> z = x < half ? box_muller<R>(w.x, w.y) : box_muller<R>(w.z, w.w);
+  if (a.noise_kind == 2) z = mk<R>(z.re * (R)1.41421356237309504880168872420969808, (R)0);
+  const R amp = (R)sqrt(h.delta) * (R)a.eta[c];
+  return mk<double>((double)(amp * z.im), (double)(-amp * z.re));
+}
 
 template <typename R>
 __device__ __forceinline__ Cx<double> gmat_pump(const GmatArgs& a, int comp, int64_t p) {
@@ -802,6 +826,10 @@ __global__ void __launch_bounds__(256) gmat_half_kernel(GmatArgs a) {
     const Cx<double> w0 = v[0] + F[0] * fn, w1 = v[1] + F[1] * fn;
     v[0] = E[0] * w0 + E[2] * w1 + F[0] * fx;
     v[1] = E[1] * w0 + E[3] * w1 + F[1] * fx;
+    if (a.noise_kind) {
+      v[0] = v[0] + gmat_noise<R>(a, h, 0, p);
+      v[1] = v[1] + gmat_noise<R>(a, h, 1, p);
+    }
   };
   if (a.flags & kEnd) half(a.h_end);
   if (a.frame[0]) {

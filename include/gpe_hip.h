@@ -101,9 +101,9 @@ int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double* c);
  * LinearAlgebra's cis of a matrix), in the descriptor form
  *     G_ab(u) = c[a + 2b] + sum_j g[(a + 2b) + 4j] |u_j|^2     (complex FP64).
  * Replaces the vector form until gpe_set_nonlinearity is called again.
- * No noise or potential (GPE_ERR_UNSUPPORTED from gpe_solve otherwise); the
- * half-steps run in a per-point kernel between transform-only passes (2D) or
- * k-space-only whole steps (1D). */
+ * No potential (GPE_ERR_UNSUPPORTED from gpe_solve otherwise); noise is drawn
+ * as for the vector form.  The half-steps run in a per-point kernel between
+ * transform-only passes (2D) or k-space-only whole steps (1D). */
 int gpe_set_nonlinearity_matrix(gpe_handle* h, const double* g, const double* c);
 
 /* Pump F(r,t) = profile(r) * envelope(t) (src/misc.jl:22-42).

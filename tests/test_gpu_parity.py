@@ -543,12 +543,24 @@ def test_matrix_nonlinearity_1d_batched_matches_oracle(ggp, orc, dtype, tol):
     print(f"matrix G 1D {np.dtype(dtype).name}: rel L2 {err:.3e}")
 
 
+@pytest.mark.parametrize("noise_dtype,shape", [(np.complex128, (64, 64)), (np.float64, (128, 4))])
+def test_matrix_nonlinearity_with_noise_matches_oracle(ggp, orc, noise_dtype, shape):
+    """Philox noise with a matrix G, drawn with the fused kernels' layout, in
+    2D (complex noise) and 1D batched (real noise)."""
+    nd = 2 if shape == (64, 64) else 1
+    L = 12.0
+    rng = np.random.default_rng(29)
+    u0 = tuple(0.3 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(2))
+    out = run_both(ggp, orc, u0, (L,) * nd, (0, 0.4), 0.02, 4, seed=77, noise_prototype=True,
+                   noise_dtype=noise_dtype, position_noise_func=lambda u, r, p: [0.2, 0.1], **rabi_terms(L))
+    err = check(*out, FP64_TOL)
+    print(f"matrix G + noise {nd}D: rel L2 {err:.3e}")
+
+
 def test_matrix_nonlinearity_limits(ggp):
-    """Noise with a matrix G is refused, not approximated."""
+    """A potential with a matrix G is refused, not approximated."""
     L = 12.0
     u0 = (np.zeros((32, 32), complex), np.zeros((32, 32), complex))
-    terms = rabi_terms(L)
-    noisy = ggp.GrossPitaevskiiProblem(u0, (L, L), position_noise_func=lambda u, r, p: [0.1, 0.1],
-                                       noise_prototype=tuple(np.empty((32, 32), complex) for _ in range(2)), **terms)
+    prob = ggp.GrossPitaevskiiProblem(u0, (L, L), potential=lambda r, p: 0.1 * r[0], **rabi_terms(L))
     with pytest.raises(ggp.GpeUnsupported):
-        ggp.solve(noisy, ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1, rng=ggp.PhiloxRNG(3))
+        ggp.solve(prob, ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1)
