@@ -1017,8 +1017,6 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     }
     HIPCHK(hipMemcpyAsync(h->d_env, h->env.data(), need * sizeof(double), hipMemcpyHostToDevice, h->stream));
   }
-  if (h->gmat && h->pot_form != GPE_FORM_NONE)
-    return fail(GPE_ERR_UNSUPPORTED, "gpe_solve: a matrix nonlinearity runs without a potential");
   const bool graph = use_step_graph(h, sps, nsaves);
   if (h->schedule == GPE_SCHEDULE_GRAPH && !graph)
     return fail(GPE_ERR_UNSUPPORTED,
@@ -1150,6 +1148,8 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     ga.nsp = h->nsp;
     ga.pump_kind = h->pump_kind;
     ga.noise_kind = h->noise_kind;
+    ga.pot_kind = h->pot_form == GPE_FORM_NONE ? 0 : (h->pot_form == GPE_FORM_SCALAR ? 1 : 2);
+    for (int c = 0; c < 2; ++c) ga.pot[c] = h->ev[c];
     for (int c = 0; c < 2; ++c) ga.eta[c] = h->eta[c];
     ga.seed_lo = (uint32_t)h->seed;
     ga.seed_hi = (uint32_t)(h->seed >> 32);

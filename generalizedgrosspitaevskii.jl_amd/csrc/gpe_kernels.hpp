@@ -762,6 +762,8 @@ struct GmatArgs {
   int noise_kind;       // 0 none, 1 complex, 2 real (as PointTerms)
   double eta[kMaxM];
   uint32_t seed_lo, seed_hi;
+  int pot_kind;         // 0 none, 1 scalar V, 2 SVector V
+  const void* pot[kMaxM];  // exp(-i dt/2 V) planes
 };
 
 // The Wiener increment of component c at flat point p of a half-step, drawn
@@ -824,8 +826,25 @@ __global__ void __launch_bounds__(256) gmat_half_kernel(GmatArgs a) {
     expm2x2_d(A, E);
     const Cx<double> fn = mk<double>(h.fnow[0], h.fnow[1]), fx = mk<double>(h.fnext[0], h.fnext[1]);
     const Cx<double> w0 = v[0] + F[0] * fn, w1 = v[1] + F[1] * fn;
-    v[0] = E[0] * w0 + E[2] * w1 + F[0] * fx;
-    v[1] = E[1] * w0 + E[3] * w1 + F[1] * fx;
+    // _mul(cis(-delta G), exp(-i delta V)) of src/kernels.jl:9-15,44: a
+    // scalar V scales the matrix; an SVector V makes it the vector E pv, which
+    // then multiplies the fields elementwise (SVector * SVector)
+    const int64_t sp = p % a.nsp;
+    if (a.pot_kind == 2) {
+      const Cx<R> q0 = ldc<R>(a.pot[0], sp), q1 = ldc<R>(a.pot[1], sp);
+      const Cx<double> p0 = mk<double>((double)q0.re, (double)q0.im), p1 = mk<double>((double)q1.re, (double)q1.im);
+      const Cx<double> e0 = E[0] * p0 + E[2] * p1, e1 = E[1] * p0 + E[3] * p1;
+      v[0] = e0 * w0 + F[0] * fx;
+      v[1] = e1 * w1 + F[1] * fx;
+    } else {
+      if (a.pot_kind == 1) {
+        const Cx<R> q = ldc<R>(a.pot[0], sp);
+        const Cx<double> sv = mk<double>((double)q.re, (double)q.im);
+        for (int i = 0; i < 4; ++i) E[i] = E[i] * sv;
+      }
+      v[0] = E[0] * w0 + E[2] * w1 + F[0] * fx;
+      v[1] = E[1] * w0 + E[3] * w1 + F[1] * fx;
+    }
     if (a.noise_kind) {
       v[0] = v[0] + gmat_noise<R>(a, h, 0, p);
       v[1] = v[1] + gmat_noise<R>(a, h, 1, p);

@@ -101,8 +101,10 @@ int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double* c);
  * LinearAlgebra's cis of a matrix), in the descriptor form
  *     G_ab(u) = c[a + 2b] + sum_j g[(a + 2b) + 4j] |u_j|^2     (complex FP64).
  * Replaces the vector form until gpe_set_nonlinearity is called again.
- * No potential (GPE_ERR_UNSUPPORTED from gpe_solve otherwise); noise is drawn
- * as for the vector form.  The half-steps run in a per-point kernel between
+ * With a potential the reference's product rules apply (src/kernels.jl:9-15):
+ * a scalar V scales the matrix, an SVector V gives the vector
+ * exp(-i dt/2 G) exp(-i dt/2 V), applied elementwise.  Noise is drawn as for
+ * the vector form.  The half-steps run in a per-point kernel between
  * transform-only passes (2D) or k-space-only whole steps (1D). */
 int gpe_set_nonlinearity_matrix(gpe_handle* h, const double* g, const double* c);
 

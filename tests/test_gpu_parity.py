@@ -557,10 +557,18 @@ def test_matrix_nonlinearity_with_noise_matches_oracle(ggp, orc, noise_dtype, sh
     print(f"matrix G + noise {nd}D: rel L2 {err:.3e}")
 
 
-def test_matrix_nonlinearity_limits(ggp):
-    """A potential with a matrix G is refused, not approximated."""
-    L = 12.0
-    u0 = (np.zeros((32, 32), complex), np.zeros((32, 32), complex))
-    prob = ggp.GrossPitaevskiiProblem(u0, (L, L), potential=lambda r, p: 0.1 * r[0], **rabi_terms(L))
-    with pytest.raises(ggp.GpeUnsupported):
-        ggp.solve(prob, ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1)
+@pytest.mark.parametrize("form", ["scalar", "svector"])
+def test_matrix_nonlinearity_with_potential_matches_oracle(ggp, orc, form):
+    """A potential with a matrix G follows the reference's product rules: a
+    scalar V scales exp(-i dt/2 G); an SVector V turns it into a vector that
+    multiplies the fields elementwise."""
+    N, L = 64, 12.0
+    rng = np.random.default_rng(31)
+    u0 = tuple(0.3 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) for _ in range(2))
+    if form == "scalar":
+        pot = lambda r, p: 0.05 * ((r[0] - 6) ** 2 + (r[1] - 6) ** 2)
+    else:
+        pot = lambda r, p: [0.05 * (r[0] - 6) ** 2, 0.03 * (r[1] - 6) ** 2 - 0.01j]
+    out = run_both(ggp, orc, u0, (L, L), (0, 0.4), 0.02, 4, potential=pot, **rabi_terms(L))
+    err = check(*out, FP64_TOL)
+    print(f"matrix G + {form} V: rel L2 {err:.3e}")

@@ -442,3 +442,33 @@ def test_matrix_nonlinearity_constant_rabi_closed_form(orc):
     want = np.einsum("ab,bxy->axy", E, np.stack(u0))
     got = np.stack([res[c][..., -1] for c in range(2)])
     assert np.max(abs(got - want)) < 1e-12 * np.max(abs(want))
+
+
+@pytest.mark.parametrize("form", ["scalar", "svector"])
+def test_matrix_nonlinearity_with_constant_potential_closed_form(orc, form):
+    """A constant non-symmetric 2x2 SMatrix G with a constant potential and no
+    dispersion.  Julia's product rules (src/kernels.jl:9-15) fix the half-step
+    factor: a scalar V gives the matrix exp(-i dt/2 G) exp(-i dt/2 V), so the
+    field after time t is exp(-i t G) exp(-i t V) u0; an SVector V gives the
+    vector e = exp(-i dt/2 G) exp(-i dt/2 V) (a matrix-vector product) that
+    multiplies each component, so u_c(t) = e_c^(2 n) u_c(0) after n steps.
+    G01 != G10 makes a transposed product fail."""
+    G = np.array([[0.3 - 0.05j, 0.7], [0.2 + 0.1j, -0.2]])
+    rng = np.random.default_rng(5)
+    u0 = tuple(rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8)) for _ in range(2))
+    V = 0.4 if form == "scalar" else [0.4, -0.3 - 0.02j]
+    prob = orc.Problem(u0, (4.0, 4.0), nonlinearity=lambda u, p: [[G[0, 0] + 0 * u[0], G[0, 1] + 0 * u[0]],
+                                                                   [G[1, 0] + 0 * u[0], G[1, 1] + 0 * u[0]]],
+                       potential=(lambda r, p: V + 0 * r[0]) if form == "scalar"
+                       else (lambda r, p: [V[0] + 0 * r[0], V[1] + 0 * r[0]]))
+    dt, T = 0.05, 1.0
+    ts, res, _ = orc.solve(prob, (0, T), dt, 2)
+    got = np.stack([res[c][..., -1] for c in range(2)])
+    if form == "scalar":
+        E = scipy.linalg.expm(-1j * ts[-1] * G) * np.exp(-1j * ts[-1] * V)
+        want = np.einsum("ab,bxy->axy", E, np.stack(u0))
+    else:
+        n = int(round(ts[-1] / dt))
+        e = scipy.linalg.expm(-0.5j * dt * G) @ np.exp(-0.5j * dt * np.asarray(V))
+        want = (e ** (2 * n))[:, None, None] * np.stack(u0)
+    assert np.max(abs(got - want)) < 1e-11 * np.max(abs(want))
