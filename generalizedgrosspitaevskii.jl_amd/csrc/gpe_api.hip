@@ -59,7 +59,10 @@ constexpr int kPlanMaxCand = 6;  // (rows, columns) pairs a measured plan times
 struct gpe_handle {
   gpe_problem_desc d;
   int M = 1, ndim = 1;
+  // n1 is the product of the extents beyond dim 0 (the x-pass's lines per
+  // grid); gy, gz are the extents of dims 1 and 2 (1 where absent)
   int64_t n0 = 0, n1 = 1, nsp = 0, batch = 1, npts = 0;
+  int64_t gy = 1, gz = 1;
   bool f64 = false;
   size_t csz = 8;
   hipStream_t stream = nullptr;
@@ -159,6 +162,9 @@ static hipError_t run_y_c(gpe_handle* h, const YPassArgs& a, int c) {
 }
 static hipError_t run_x(gpe_handle* h, const XPassArgs& a) { return run_x_w(h, a, h->plan_w); }
 static hipError_t run_y(gpe_handle* h, const YPassArgs& a) { return run_y_c(h, a, h->plan_c); }
+static hipError_t run_col(gpe_handle* h, const YPassArgs& a, int64_t n) {
+  return h->f64 ? gpe_run_y_f64(h->M, n, a, 0, h->stream) : gpe_run_y_f32(h->M, n, a, 0, h->stream);
+}
 static hipError_t run_1d(gpe_handle* h, const Step1DArgs& a) {
   return h->f64 ? gpe_run_1d_f64(h->M, h->n0, a, h->stream) : gpe_run_1d_f32(h->M, h->n0, a, h->stream);
 }
@@ -246,7 +252,7 @@ extern "C" int gpe_create(const gpe_problem_desc* desc, gpe_handle** out) {
   if (!desc || !out) return fail(GPE_ERR_INVALID, "gpe_create: null argument");
   *out = nullptr;
   const gpe_problem_desc& d = *desc;
-  if (d.ndim < 1 || d.ndim > 2) return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ndim=%d (supported: 1, 2)", d.ndim);
+  if (d.ndim < 1 || d.ndim > 3) return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ndim=%d (supported: 1, 2, 3)", d.ndim);
   if (d.ncomp < 1 || d.ncomp > kMaxM) return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ncomp=%d (supported: 1, 2)", d.ncomp);
   if (d.dtype != GPE_C64 && d.dtype != GPE_C128) return fail(GPE_ERR_INVALID, "gpe_create: dtype=%d", d.dtype);
   if (d.batch < 1) return fail(GPE_ERR_INVALID, "gpe_create: batch=%lld", (long long)d.batch);
@@ -269,7 +275,9 @@ extern "C" int gpe_create(const gpe_problem_desc* desc, gpe_handle** out) {
   h->M = d.ncomp;
   h->ndim = d.ndim;
   h->n0 = d.n[0];
-  h->n1 = d.ndim == 2 ? d.n[1] : 1;
+  h->gy = d.ndim >= 2 ? d.n[1] : 1;
+  h->gz = d.ndim == 3 ? d.n[2] : 1;
+  h->n1 = h->gy * h->gz;
   h->nsp = h->n0 * h->n1;
   h->batch = d.batch;
   h->npts = h->nsp * h->batch;
@@ -285,7 +293,7 @@ extern "C" int gpe_create(const gpe_problem_desc* desc, gpe_handle** out) {
   for (int c = 0; c < h->M; ++c)
     if (hipMalloc(&h->u[c], field_bytes(h)) != hipSuccess)
       return cleanup(fail(GPE_ERR_NOMEM, "gpe_create: field allocation of %zu bytes", field_bytes(h)));
-  h->ntw = std::max(h->n0, h->n1);
+  h->ntw = std::max(h->n0, std::max(h->gy, h->gz));
   std::vector<double> w;
   twiddles_fp64(h->ntw, w);
   if (h->f64) {
@@ -499,6 +507,8 @@ extern "C" int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, 
   h->pump_kind = GPE_PUMP_NONE;
   if (kind == GPE_PUMP_NONE) return GPE_OK;
   if (kind != GPE_PUMP_TABLE && kind != GPE_PUMP_SEPARABLE) return fail(GPE_ERR_INVALID, "gpe_set_pump: kind=%d", kind);
+  if (kind == GPE_PUMP_SEPARABLE && h->ndim == 3)
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_set_pump: 3D grids take the table form");
   if (!profile) return fail(GPE_ERR_INVALID, "gpe_set_pump: null profile");
   if (n_env < 0 || (n_env > 0 && !envelope)) return fail(GPE_ERR_INVALID, "gpe_set_pump: envelope");
   const int64_t per = kind == GPE_PUMP_TABLE ? h->nsp : (h->n0 + (h->ndim == 2 ? h->n1 : 0));
@@ -699,6 +709,40 @@ static YPassArgs y_args(const gpe_handle* h) {
   a.tw = h->tw;
   a.tws = (int)(h->ntw / h->n1);
   return a;
+}
+
+// Column passes of a 3D grid (x fastest, then y, then z), with the built-in
+// column count: which = 1 FFT_y, 2 IFFT_y (columns of every (z, trajectory)
+// slab), 3 FFT_z -> exp(-i dt D) / prod(N) -> IFFT_z (columns over (x, y)).
+static YPassArgs col_args_3d(const gpe_handle* h, int which) {
+  YPassArgs a;
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < kMaxM; ++c) a.u[c] = h->u[c];
+  const int64_t n = which == 3 ? h->gz : h->gy;
+  const int C = y_cols_per_wg((int)n);
+  a.nx = which == 3 ? h->n0 * h->gy : h->n0;
+  a.ncolblk = a.nx / C;
+  a.nblocks = a.ncolblk * (which == 3 ? h->batch : h->gz * h->batch);
+  a.xcd_remap = (a.nblocks % 8 == 0) ? 1 : 0;
+  a.mode = which == 3 ? 0 : which;
+  if (which == 3) {
+    a.ed = ed_desc(h);
+    a.kx_n = h->n0;
+    a.ky_n = h->gy;
+  }
+  a.tw = h->tw;
+  a.tws = (int)(h->ntw / n);
+  return a;
+}
+
+// The k-space part of a 2D or 3D step: the y-pass, or the three column
+// passes of a 3D grid.
+static hipError_t run_k(gpe_handle* h, const YPassArgs& ya) {
+  if (h->ndim != 3) return run_y(h, ya);
+  hipError_t e = run_col(h, col_args_3d(h, 1), h->gy);
+  if (e == hipSuccess) e = run_col(h, col_args_3d(h, 3), h->gz);
+  if (e == hipSuccess) e = run_col(h, col_args_3d(h, 2), h->gy);
+  return e;
 }
 
 static Step1DArgs s1_args(gpe_handle* h, double dt) {
@@ -1136,6 +1180,7 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     XPassArgs xa = x_args(h);
     // (1D handles have no column pass: y_args would divide by n1 / 8 = 0)
     const YPassArgs ya = h->ndim == 2 ? y_args(h) : YPassArgs{};
+    const bool grid = h->ndim >= 2;
     GmatArgs ga;
     memset(&ga, 0, sizeof ga);
     for (int c = 0; c < 2; ++c) {
@@ -1173,14 +1218,14 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
     ga.flags = kStart;
     ga.h_start = make_half(h, rdt, 1, 0, call0);
     HIPCHK(run_gm());
-    if (h->ndim == 2) {
+    if (grid) {
       xa.flags = kOutK;
       HIPCHK(run_x(h, xa));
     }
     for (int64_t s = 1; s <= S; ++s) {
       t += rdt;
-      if (h->ndim == 2) {
-        HIPCHK(run_y(h, ya));
+      if (grid) {
+        HIPCHK(run_k(h, ya));
         xa.flags = kInK;
         HIPCHK(run_x(h, xa));
       } else {
@@ -1193,7 +1238,7 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
       ga.h_end = make_half(h, rdt, s, 1, call0);
       if (s < S) ga.h_start = make_half(h, rdt, s + 1, 0, call0);
       HIPCHK(run_gm());
-      if (s < S && h->ndim == 2) {
+      if (s < S && grid) {
         xa.flags = kOutK;
         HIPCHK(run_x(h, xa));
       }
@@ -1203,15 +1248,15 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
         if (ts) ts[s / sps - (save_start ? 0 : 1)] = t;
       }
     }
-  } else if (h->ndim == 2) {
+  } else if (h->ndim >= 2) {
     XPassArgs xa = x_args(h);
-    const YPassArgs ya = y_args(h);
+    const YPassArgs ya = h->ndim == 2 ? y_args(h) : YPassArgs{};
     xa.flags = kStart | kOutK;
     xa.h_start = make_half(h, rdt, 1, 0, call0);
     HIPCHK(run_x(h, xa));
     for (int64_t s = 1; s <= S; ++s) {
       t += rdt;
-      HIPCHK(run_y(h, ya));
+      HIPCHK(run_k(h, ya));
       const bool save = (s % sps) == 0;
       const int64_t f = s / sps - 1 + (save_start ? 1 : 0);
       xa.h_end = make_half(h, rdt, s, 1, call0);
@@ -1255,7 +1300,7 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
   HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
   h->configured = true;
   h->dt = rdt;
-  if (h->ndim == 2) h->last_schedule = graph ? GPE_SCHEDULE_GRAPH : GPE_SCHEDULE_LAUNCHES;
+  if (h->ndim >= 2) h->last_schedule = graph ? GPE_SCHEDULE_GRAPH : GPE_SCHEDULE_LAUNCHES;
   return GPE_OK;
 }
 
@@ -1455,14 +1500,14 @@ static int advance_impl(gpe_handle* h, int64_t nsteps, bool per_kernel, double* 
   int rc = GPE_OK;
   const uint64_t call0 = h->call;
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  if (h->ndim == 2) {
+  if (h->ndim >= 2) {
     XPassArgs xa = x_args(h);
-    const YPassArgs ya = y_args(h);
+    const YPassArgs ya = h->ndim == 2 ? y_args(h) : YPassArgs{};
     xa.flags = kStart | kOutK;
     xa.h_start = make_half(h, h->dt, 1, 0, call0);
     rc = launch(0, [&] { return run_x(h, xa); });
     for (int64_t s = 1; s <= nsteps && rc == GPE_OK; ++s) {
-      rc = launch(1, [&] { return run_y(h, ya); });
+      rc = launch(1, [&] { return run_k(h, ya); });
       if (rc) break;
       xa.h_end = make_half(h, h->dt, s, 1, call0);
       if (s < nsteps) {
@@ -1514,7 +1559,7 @@ static int advance_impl(gpe_handle* h, int64_t nsteps, bool per_kernel, double* 
     for (auto& e : pe) (void)hipEventDestroy(e);
     empty /= npair;
     h->event_pair_ms = empty;
-    if (ms_x) *ms_x = h->ndim == 2 ? (cnt[0] ? acc[0] / cnt[0] : 0) : (cnt[2] ? acc[2] / cnt[2] : 0);
+    if (ms_x) *ms_x = h->ndim >= 2 ? (cnt[0] ? acc[0] / cnt[0] : 0) : (cnt[2] ? acc[2] / cnt[2] : 0);
     if (ms_y) *ms_y = cnt[1] ? acc[1] / cnt[1] : 0;
   }
   return GPE_OK;
@@ -1927,15 +1972,15 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
   int64_t s = 0, nh = 0;
   double r = INFINITY;
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  if (h->ndim == 2) {
+  if (h->ndim >= 2) {
     XPassArgs xa = x_args(h);
-    const YPassArgs ya = y_args(h);
+    const YPassArgs ya = h->ndim == 2 ? y_args(h) : YPassArgs{};
     // the steps of a check block before the two that write the compared
     // fields are identical without noise: one prebuilt graph replays them
     hipGraphExec_t gexec = nullptr;
     GraphExecGuard exec_guard{gexec, h->stream};
-    const bool graph =
-        h->noise_kind == GPE_NOISE_NONE && check_every >= 4 && h->schedule != GPE_SCHEDULE_LAUNCHES;
+    const bool graph = h->ndim == 2 && h->noise_kind == GPE_NOISE_NONE && check_every >= 4 &&
+                       h->schedule != GPE_SCHEDULE_LAUNCHES;
     if (graph) {
       XPassArgs xg = x_args(h);
       xg.flags = kInK | kEnd | kStart | kOutK;
@@ -1957,7 +2002,7 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
         s += check_every - 2;
       }
       ++s;
-      if (run_y(h, ya) != hipSuccess) {
+      if (run_k(h, ya) != hipSuccess) {
         rc = fail(GPE_ERR_DEVICE, "gpe_steady_state: launch");
         break;
       }

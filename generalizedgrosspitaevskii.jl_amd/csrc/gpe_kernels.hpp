@@ -8,6 +8,8 @@
 //        y-pass   FFT_y -> exp(-i dt D(kx,ky)) / (Nx Ny) -> IFFT_y
 //        The real-space half-steps that end step s and start step s+1 are
 //        pointwise and adjacent, so one x-pass applies both.
+//   3D:  x-pass as in 2D; y-pass FFT_y only; z-pass FFT_z ->
+//        exp(-i dt D(kx,ky,kz)) / (Nx Ny Nz) -> IFFT_z; y-pass IFFT_y only.
 //   1D:  whole-step kernel: every line stays in registers for a block of
 //        steps (half, FFT, exp(-i dt D), IFFT, half, repeated).
 //
@@ -463,10 +465,12 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
 // ------------------------------------------------------------------ y-pass
 struct YPassArgs {
   void* u[kMaxM];
-  int64_t nx;         // dim-0 extent (columns)
+  int64_t nx;         // columns per grid (dim-0 extent; dims 0 and 1 of a 3D grid's z-pass)
   int64_t ncolblk;    // nx / C
   int64_t nblocks;    // ncolblk * batch
   int xcd_remap;      // 1: give each XCD a contiguous range of column blocks
+  int mode;           // 0: FFT, exp(-i dt D), IFFT; 1: FFT only; 2: IFFT only
+  int64_t kx_n, ky_n; // z-pass of a 3D grid: column = kx + kx_n ky, table k1 = ky + ky_n kz (0 otherwise)
   EdDesc ed;
   const void* tw;
   int tws;
@@ -505,12 +509,29 @@ __global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
   TwReg<R, N> tr;
   load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
   EdVals<R> ev;
-  load_ed<R, M>(ev, a.ed, c, x0 + cx, 0, t, T);
+  if (a.kx_n) {
+    // z-pass of a 3D grid: the table holds (kx, ky + ny kz), kz fastest
+    const int64_t col = x0 + cx;
+    load_ed<R, M>(ev, a.ed, c, col % a.kx_n, 0, col / a.kx_n + a.ky_n * t, a.ky_n * T);
+  } else if (a.mode == 0) {
+    load_ed<R, M>(ev, a.ed, c, x0 + cx, 0, t, T);
+  }
   Cx<R>* my = lds + (c * C + cx) * LS;
   const Cx<R>* partner = lds + ((M - 1 - c) * C + cx) * LS;
   const int boff = M * C * LS;
   // a component's C columns interleave over C*T consecutive threads
   constexpr bool LOCAL = C * T <= 64;
+  if (a.mode != 0) {
+    // transform-only pass (the y direction of a 3D step, on either side of
+    // the z-pass); the mode is uniform over the launch
+    if (a.mode == 1)
+      fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
+    else
+      fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
+#pragma unroll
+    for (int m = 0; m < kE; ++m) uc[(int64_t)(t + m * T) * nx] = v[m];
+    return;
+  }
   // buffer read by the forward transform's last exchange; the other one is
   // free once that transform ends, so the components exchange through it and
   // the inverse transform starts in this one (no barrier between them)

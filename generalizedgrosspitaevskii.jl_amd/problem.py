@@ -488,8 +488,8 @@ class StrangSplittingIterator:
         u0 = prob.u0
         shape = tuple(u0[0].shape)
         N, M = prob.N, prob.M
-        if N > 2:
-            raise NotImplementedError("libgpe_hip supports 1D and 2D grids")
+        if N > 3:
+            raise NotImplementedError("libgpe_hip supports 1D, 2D and 3D grids")
         spatial = shape[:N]
         self.shape, self.spatial = shape, spatial
         batch = int(np.prod(shape[N:])) if len(shape) > N else 1

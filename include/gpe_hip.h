@@ -48,7 +48,7 @@ enum gpe_pump_kind { GPE_PUMP_NONE = 0, GPE_PUMP_TABLE = 1, GPE_PUMP_SEPARABLE =
 enum gpe_noise_kind { GPE_NOISE_NONE = 0, GPE_NOISE_COMPLEX = 1, GPE_NOISE_REAL = 2 };
 
 typedef struct gpe_problem_desc {
-  int32_t ndim;   /* spatial dimensions: 1 or 2 (length(lengths))             */
+  int32_t ndim;   /* spatial dimensions: 1, 2 or 3 (length(lengths))          */
   int32_t ncomp;  /* M components: 1 or 2 (length(u0))                        */
   int64_t n[3];   /* grid points per spatial dimension (powers of two, 16..4096) */
   double L[3];    /* box lengths (prob.lengths)                               */
@@ -105,12 +105,12 @@ int gpe_set_nonlinearity(gpe_handle* h, const double* g, const double* c);
  * a scalar V scales the matrix, an SVector V gives the vector
  * exp(-i dt/2 G) exp(-i dt/2 V), applied elementwise.  Noise is drawn as for
  * the vector form.  The half-steps run in a per-point kernel between
- * transform-only passes (2D) or k-space-only whole steps (1D). */
+ * transform-only passes (2D, 3D) or k-space-only whole steps (1D). */
 int gpe_set_nonlinearity_matrix(gpe_handle* h, const double* g, const double* c);
 
 /* Pump F(r,t) = profile(r) * envelope(t) (src/misc.jl:22-42).
  *   GPE_PUMP_TABLE:     profile = M blocks of prod(n) complex values.
- *   GPE_PUMP_SEPARABLE: profile = M blocks of (n[0] + ... + n[ndim-1])
+ *   GPE_PUMP_SEPARABLE (1D and 2D): profile = M blocks of (n[0] + ... + n[ndim-1])
  *                       complex values; F_c(r) = prod_d p_{c,d}[i_d].
  * envelope: n_env complex FP64 samples at the times gpe_pump_times() lists
  * for the next gpe_solve; n_env == 0 means envelope == 1 (static pump). */

@@ -134,3 +134,24 @@ def test_wigner_density_g2_closed_forms(ggp):
     nbar = 2.0 * c
     n, g2 = ggp.wigner_density_g2(*moments(cgauss(nbar + c / 2)), c)
     assert abs(n / nbar - 1) < 0.02 and abs(g2 - 2) < 0.05
+
+
+def test_pump_descriptor_3d_is_a_column_major_table(ggp):
+    """3D pumps go to the library as a full table (the separable form has
+    per-axis factors for 1D and 2D only), flattened column-major like the
+    field: entry (i, j, k) of component c at c * prod(n) + i + n0 (j + n1 k)."""
+    from gpe_hip_amd import _lib as Lb
+    from gpe_hip_amd import problem as P
+
+    n, L = (16, 8, 4), (4.0, 2.0, 1.0)
+    f = lambda r, q, t: [np.exp(-((r[0] - 2) ** 2 + 3 * (r[1] - 1) ** 2 + 5 * (r[2] - 0.5) ** 2)), 0 * r[0]]
+    p = ggp.GrossPitaevskiiProblem((np.zeros(n),) * 2, L, pump=f)
+    rax = P._bcast_axes(ggp.direct_grid(p))
+    kind, prof, env = P.pump_descriptor(p, rax, n, np.linspace(0, 1, 9))
+    assert kind == Lb.GPE_PUMP_TABLE and env is None
+    tab = prof.view(np.complex128)
+    assert tab.size == 2 * int(np.prod(n))
+    want = f(tuple(np.meshgrid(*[np.arange(m) * (l / m) for m, l in zip(n, L)], indexing="ij")), None, 0)[0]
+    i, j, k = 5, 3, 2
+    assert tab[i + n[0] * (j + n[1] * k)] == want[i, j, k]
+    assert not np.any(tab[int(np.prod(n)):])

@@ -472,3 +472,50 @@ def test_matrix_nonlinearity_with_constant_potential_closed_form(orc, form):
         e = scipy.linalg.expm(-0.5j * dt * G) @ np.exp(-0.5j * dt * np.asarray(V))
         want = (e ** (2 * n))[:, None, None] * np.stack(u0)
     assert np.max(abs(got - want)) < 1e-11 * np.max(abs(want))
+
+
+# ------------------------------------------------------------------ 3D grids
+def test_free_propagation_3d_exact():
+    """3D grid, anisotropic D = a k_x^2 + b k_y^2 + c k_z^2 on a non-cubic box
+    (docs/src/spatial_grid.md:17-18 lists 3D domains): Strang is the exact
+    spectral propagator, checked with scipy's FFT.  Distinct extents, lengths
+    and coefficients per axis make a swapped axis fail."""
+    import scipy.fft
+
+    n, Ls, co = (16, 8, 32), (3.0, 5.0, 7.0), (0.5, 0.3, 0.2)
+    rng = np.random.default_rng(21)
+    u0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    disp = lambda k, p: co[0] * k[0] ** 2 + co[1] * k[1] ** 2 + co[2] * k[2] ** 2
+    ts, res, _ = orc.solve(orc.Problem((u0,), Ls, dispersion=disp), (0, 0.5), 0.05, 2)
+    ks = [orc.reciprocal_grid_1d(L, m) for L, m in zip(Ls, n)]
+    D = co[0] * ks[0][:, None, None] ** 2 + co[1] * ks[1][None, :, None] ** 2 + co[2] * ks[2][None, None, :] ** 2
+    U0 = scipy.fft.fftn(u0)
+    for f, t in enumerate(ts):
+        exact = scipy.fft.ifftn(np.exp(-1j * D * t) * U0)
+        assert np.linalg.norm(res[0][..., f] - exact) <= 1e-12 * np.linalg.norm(exact)
+
+
+def test_two_component_plane_wave_3d_exact():
+    """A single 3D plane wave exp(i q.r) in both components under a 2x2
+    non-symmetric D(k) and no real-space terms: the mode evolves as
+    expm(-i t D(q)) (scipy), and every other mode stays empty."""
+    n, Ls = (8, 16, 8), (2.0, 4.0, 3.0)
+    m = (1, 3, 6)  # mode indices (FFT order; 6 of 8 is the negative -2)
+    ks = [orc.reciprocal_grid_1d(L, N) for L, N in zip(Ls, n)]
+    q = np.array([ks[d][m[d]] for d in range(3)])
+    r = np.meshgrid(*[np.arange(N) * (L / N) for L, N in zip(Ls, n)], indexing="ij")
+    wave = np.exp(1j * (q[0] * r[0] + q[1] * r[1] + q[2] * r[2]))
+    a0 = np.array([0.6 - 0.1j, 0.2 + 0.5j])
+    u0 = (a0[0] * wave, a0[1] * wave)
+
+    def disp(k, p):
+        kk = k[0] ** 2 + k[1] ** 2 + k[2] ** 2
+        return [[kk / 2 - 0.1j, 0.3 * k[2] + 0.2], [0.4 - 0.05j * k[0], kk / 3]]
+
+    ts, res, _ = orc.solve(orc.Problem(u0, Ls, dispersion=disp), (0, 1.0), 0.1, 2)
+    kk = float(q @ q)
+    Dq = np.array([[kk / 2 - 0.1j, 0.3 * q[2] + 0.2], [0.4 - 0.05j * q[0], kk / 3]])
+    for f, t in enumerate(ts):
+        a = scipy.linalg.expm(-1j * t * Dq) @ a0
+        for c in range(2):
+            np.testing.assert_allclose(res[c][..., f], a[c] * wave, atol=1e-13)
