@@ -520,6 +520,45 @@ def size_sweep(g, torch, sizes=(256, 512, 1024, 2048, 4096), target_s=0.5):
     return {"metric": "grid-point-steps/s by grid size (two-component polariton, FP32)", "rows": rows}
 
 
+def grid3d_run(g, torch, n=128, target_s=0.5):
+    """The polariton terms on an n^3 grid (two components, FP32 fields, box
+    L = 100 n / 512 per axis, frames every 100 steps into device memory).  A 3D
+    step is four passes (x-pass, FFT_y, the z-pass with exp(-i dt D), IFFT_y):
+    128 B per grid-point-step at FP32."""
+    L = L_BOX * n / N_GRID
+    shape = (n, n, n)
+    u0 = []
+    for _ in range(2):
+        t = g.device_empty(shape, "complex64", 0)
+        t.zero_()
+        u0.append(t)
+    p = polariton_param()
+    p.update(L=L)
+    prob = g.GrossPitaevskiiProblem(tuple(u0), (L, L, L), dispersion=dispersion, nonlinearity=nonlinearity,
+                                    pump=pump, param=p)
+    it = g.init(prob, g.StrangSplitting(), (0.0, 100 * 0.05), dt=0.05, nsaves=1)
+    it.solve(save_frames=False)
+    torch.cuda.synchronize()
+    est = max(it.last_ms() / 100, 1e-3)
+    it.close()
+    steps = max(100, int(target_s * 1e3 / est) // 100 * 100)
+    it = g.init(prob, g.StrangSplitting(), (0.0, steps * 0.05), dt=0.05, nsaves=steps // 100)
+    res = tuple(g.device_empty(shape + (it.nsaves + 1,), "complex64", 0) for _ in range(2))
+    it.solve(result=res)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it.solve(result=res)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    finite = bool(torch.isfinite(torch.view_as_real(res[0][..., -1])).all().item())
+    it.close()
+    gps = n ** 3 * steps / wall
+    return {"workload": f"two-component polariton on {n}^3, FP32, frames every 100 steps (4 passes per step)",
+            "value": gps, "unit": "grid-point-steps/s", "ms_per_step": wall * 1e3 / steps, "steps": steps,
+            "dtype": "c64", "bytes_per_gps": 128, "hbm_equiv_frac": gps * 128 / 1e9 / HBM_PEAK_GBS,
+            "state_mib": 2 * n ** 3 * 8 / 2 ** 20, "finite": finite}
+
+
 def fno_bench(torch, batch=16, reps=10):
     """Throughput of the warm-start network's kernels alone: the 256x256 FNO
     (width 32, 16 modes, 4 layers, random weights) forward on `batch` cases
@@ -620,6 +659,7 @@ def main():
     ap.add_argument("--no-twa", action="store_true", help="skip the batched truncated-Wigner line")
     ap.add_argument("--twa-steps", type=int, default=2000)
     ap.add_argument("--no-sweep", action="store_true", help="skip the uneven-size sweep line")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D grid line")
     ap.add_argument("--plan", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
                     help="x_rows,y_cols: fix the headline's launch shapes instead of measuring them (profiling "
                          "runs use it: a profiler's per-dispatch cost skews the planner's timings)")
@@ -732,6 +772,8 @@ def main():
         other["twa_1024x256sq"] = twa_run(g, torch, args.twa_steps)
     if not args.no_sweep:
         other["sweep_4096x1d"] = sweep_run(g, torch)
+    if not args.no_3d:
+        other["polariton_128cube"] = grid3d_run(g, torch)
     if other:
         out["other_workloads"] = other
     if not args.no_cpu:
