@@ -253,7 +253,8 @@ extern "C" int gpe_create(const gpe_problem_desc* desc, gpe_handle** out) {
   *out = nullptr;
   const gpe_problem_desc& d = *desc;
   if (d.ndim < 1 || d.ndim > 3) return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ndim=%d (supported: 1, 2, 3)", d.ndim);
-  if (d.ncomp < 1 || d.ncomp > kMaxM) return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ncomp=%d (supported: 1, 2)", d.ncomp);
+  if (d.ncomp < 1 || d.ncomp > kMaxM)
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_create: ncomp=%d (supported: 1..%d)", d.ncomp, kMaxM);
   if (d.dtype != GPE_C64 && d.dtype != GPE_C128) return fail(GPE_ERR_INVALID, "gpe_create: dtype=%d", d.dtype);
   if (d.batch < 1) return fail(GPE_ERR_INVALID, "gpe_create: batch=%lld", (long long)d.batch);
   for (int i = 0; i < d.ndim; ++i) {
@@ -261,10 +262,14 @@ extern "C" int gpe_create(const gpe_problem_desc* desc, gpe_handle** out) {
       return fail(GPE_ERR_UNSUPPORTED, "gpe_create: n[%d]=%lld (supported: powers of two 16..4096)", i, (long long)d.n[i]);
     if (!(d.L[i] > 0) || !std::isfinite(d.L[i])) return fail(GPE_ERR_INVALID, "gpe_create: L[%d]=%g", i, d.L[i]);
   }
-  if (d.dtype == GPE_C128 && d.ncomp == 2) {
-    for (int i = 0; i < d.ndim; ++i)
-      if (d.n[i] > 2048)
-        return fail(GPE_ERR_UNSUPPORTED, "gpe_create: n[%d]=%lld exceeds 2048 for two FP64 components (LDS)", i, (long long)d.n[i]);
+  // a pass holds the M components of a line in one workgroup: M * N/8
+  // threads and two LDS buffers of M padded lines
+  for (int i = 0; i < d.ndim; ++i) {
+    const bool f64 = d.dtype == GPE_C128;
+    if (d.ncomp * (d.n[i] / kE) > 1024 || pass_lds_bytes((int)d.n[i], d.ncomp, f64) > 160 * 1024)
+      return fail(GPE_ERR_UNSUPPORTED, "gpe_create: n[%d]=%lld is too long for %d %s components (%s)", i,
+                  (long long)d.n[i], d.ncomp, f64 ? "FP64" : "FP32",
+                  d.ncomp * (d.n[i] / kE) > 1024 ? "threads per workgroup" : "LDS");
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GPE_ERR_DEVICE, "gpe_create: no HIP device");
@@ -356,6 +361,26 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
   if (form < GPE_FORM_SCALAR || form > GPE_FORM_FULL) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: form=%d", form);
   if (!D) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: null table");
   const int M = h->M;
+  std::vector<double> diag_only;
+  if (M > 2 && form == GPE_FORM_FULL) {
+    // the k-space pass of M > 2 components applies a diagonal exp(-i dt D):
+    // a matrix D(k) is accepted when its off-diagonal entries are zero
+    diag_only.resize((size_t)2 * M * h->nsp);
+    for (int64_t k = 0; k < h->nsp; ++k)
+      for (int a = 0; a < M; ++a)
+        for (int b = 0; b < M; ++b) {
+          const double* e = D + 2 * ((size_t)k * M * M + a + (size_t)M * b);
+          if (a == b) {
+            diag_only[2 * ((size_t)k * M + a)] = e[0];
+            diag_only[2 * ((size_t)k * M + a) + 1] = e[1];
+          } else if (e[0] != 0.0 || e[1] != 0.0) {
+            return fail(GPE_ERR_UNSUPPORTED,
+                        "gpe_set_dispersion: a non-diagonal D(k) needs M <= 2 components (M = %d)", M);
+          }
+        }
+    D = diag_only.data();
+    form = GPE_FORM_DIAG;
+  }
   const int per_in = per_point(form, M);
   const int64_t n0 = h->n0, n1 = h->n1;
   // device form and values per entry

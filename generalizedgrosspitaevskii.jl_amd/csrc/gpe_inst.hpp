@@ -164,19 +164,26 @@ inline int rowcorr_blocks(int64_t N, int64_t nlines) {
 }  // namespace gpe
 
 #define GPE_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+// Three and four components: lines up to 2048 (a workgroup holds M lines of
+// N/8 threads; 4096 would need 2048 threads).  The FP64 files stop at 1024
+// (LDS), see GPE_SIZES34_f64.
+#define GPE_SIZES34_f32(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+#define GPE_SIZES34_f64(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
 
 #define GPE_DEFINE_RUNNERS(R, SUFFIX)                                                                  \
   hipError_t gpe_run_x_##SUFFIX(int M, int64_t N, const gpe::XPassArgs& a, int64_t nlines, int w, hipStream_t s,  \
                                 gpe::LaunchDesc* d) {                                                  \
-    switch (N * 4 + M) {                                                                               \
+    switch (N * 8 + M) {                                                                               \
       GPE_SIZES(GPE_XCASE_##SUFFIX)                                                                    \
+      GPE_SIZES34_##SUFFIX(GPE_XCASE34_##SUFFIX)                                                       \
     }                                                                                                  \
     return hipErrorInvalidValue;                                                                       \
   }                                                                                                    \
   hipError_t gpe_run_y_##SUFFIX(int M, int64_t N, const gpe::YPassArgs& a, int c, hipStream_t s,       \
                                 gpe::LaunchDesc* d) {                                                  \
-    switch (N * 4 + M) {                                                                               \
+    switch (N * 8 + M) {                                                                               \
       GPE_SIZES(GPE_YCASE_##SUFFIX)                                                                    \
+      GPE_SIZES34_##SUFFIX(GPE_YCASE34_##SUFFIX)                                                       \
     }                                                                                                  \
     return hipErrorInvalidValue;                                                                       \
   }                                                                                                    \
@@ -187,8 +194,9 @@ inline int rowcorr_blocks(int64_t N, int64_t nlines) {
     return hipErrorInvalidValue;                                                                       \
   }                                                                                                    \
   hipError_t gpe_run_1d_##SUFFIX(int M, int64_t N, const gpe::Step1DArgs& a, hipStream_t s) {         \
-    switch (N * 4 + M) {                                                                               \
+    switch (N * 8 + M) {                                                                               \
       GPE_SIZES(GPE_1CASE_##SUFFIX)                                                                    \
+      GPE_SIZES34_##SUFFIX(GPE_1CASE34_##SUFFIX)                                                       \
     }                                                                                                  \
     return hipErrorInvalidValue;                                                                       \
   } \

@@ -24,7 +24,7 @@
 
 namespace gpe {
 
-constexpr int kMaxM = 2;
+constexpr int kMaxM = 4;
 
 // Parameters of one real-space half-step (src/strang_splitting.jl:78-84).
 struct HalfStep {
@@ -67,7 +67,16 @@ __device__ __forceinline__ Cx<R> cxd(const double (&a)[2]) {
 // Workgroups are laid out component-major, so every wavefront belongs to a
 // single component and component-dependent branches are wave-uniform.
 
-__device__ __forceinline__ const void* selp(int c, const void* a0, const void* a1) { return c ? a1 : a0; }
+// Entry c (< M) of a per-component argument array.  c is uniform over a
+// wavefront but not known to the compiler as such; a select chain keeps the
+// kernel arguments out of private memory.
+template <int M, typename T>
+__device__ __forceinline__ T pick(const T (&a)[kMaxM], int c) {
+  T r = a[0];
+#pragma unroll
+  for (int j = 1; j < M; ++j) r = (c == j) ? a[j] : r;
+  return r;
+}
 
 // The Wiener increment -i sqrt(delta) eta_c xi_c of the thread's E values.
 // Element m and m + E/2 (x and x + N/2 on the line) share one Philox draw:
@@ -109,16 +118,16 @@ struct PointCache {
   Cx<R> pump[kE];
   Cx<R> pot[kE];
   bool pumped, has_g;
-  R cr, ci, gr[M], gi[M], eta;
+  R cr, ci, gr[M], gi[M], gdr, gdi, eta;  // row c of g, and its diagonal g_cc
 };
 
 template <typename R, int M>
 __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PointTerms& pt, int c, int t, int T,
                                                  int64_t y, int64_t sp_off) {
-  const void* pa = selp(c, pt.pump_a[0], pt.pump_a[M - 1]);
+  const void* pa = pick<M>(pt.pump_a, c);
   pc.pumped = pt.pump_kind != 0 && pa != nullptr;
   if (pc.pumped) {
-    const void* pbp = selp(c, pt.pump_b[0], pt.pump_b[M - 1]);
+    const void* pbp = pick<M>(pt.pump_b, c);
     const Cx<R> pb = (pt.pump_kind == 2 && pbp) ? ldc<R>(pbp, y) : mk<R>((R)1, (R)0);
 #pragma unroll
     for (int m = 0; m < kE; ++m) {
@@ -127,19 +136,42 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const Poi
     }
   }
   if (pt.pot_kind) {
-    const void* pp = selp(c, pt.pot[0], pt.pot[M - 1]);
+    const void* pp = pick<M>(pt.pot, c);
 #pragma unroll
     for (int m = 0; m < kE; ++m) pc.pot[m] = ldc<R>(pp, sp_off + t + m * T);
   }
   pc.has_g = (pt.gmask >> c) & 1;
-  pc.cr = (R)(c ? pt.c[M - 1][0] : pt.c[0][0]);
-  pc.ci = (R)(c ? pt.c[M - 1][1] : pt.c[0][1]);
+  double cr = pt.c[0][0], ci = pt.c[0][1];
+  double gr[M], gi[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) {
-    pc.gr[j] = (R)(c ? pt.g[M - 1][j][0] : pt.g[0][j][0]);
-    pc.gi[j] = (R)(c ? pt.g[M - 1][j][1] : pt.g[0][j][1]);
+    gr[j] = pt.g[0][j][0];
+    gi[j] = pt.g[0][j][1];
   }
-  pc.eta = (R)(c ? pt.eta[M - 1] : pt.eta[0]);
+#pragma unroll
+  for (int i = 1; i < M; ++i) {
+    const bool me = c == i;
+    cr = me ? pt.c[i][0] : cr;
+    ci = me ? pt.c[i][1] : ci;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      gr[j] = me ? pt.g[i][j][0] : gr[j];
+      gi[j] = me ? pt.g[i][j][1] : gi[j];
+    }
+  }
+  pc.cr = (R)cr;
+  pc.ci = (R)ci;
+  double gdr = gr[0], gdi = gi[0];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    pc.gr[j] = (R)gr[j];
+    pc.gi[j] = (R)gi[j];
+    gdr = (c == j) ? gr[j] : gdr;
+    gdi = (c == j) ? gi[j] : gdi;
+  }
+  pc.gdr = (R)gdr;
+  pc.gdi = (R)gdi;
+  pc.eta = (R)pick<M>(pt.eta, c);
 }
 
 // Real-space half-step (muladd_kernel!, src/kernels.jl:37-54) on the E values
@@ -187,8 +219,8 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
             gi += pc.gi[j] * a2j[j][m];
           }
         } else {
-          gr += ((M > 1 && c) ? pc.gr[M - 1] : pc.gr[0]) * a2j[0][m];
-          gi += ((M > 1 && c) ? pc.gi[M - 1] : pc.gi[0]) * a2j[0][m];
+          gr += pc.gdr * a2j[0][m];
+          gi += pc.gdi * a2j[0][m];
         }
         R sn, co;
         fast_sincos(delta * gr, sn, co);
@@ -223,10 +255,8 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
           gi += pc.gi[j] * a2j[j][m];
         }
       } else {
-        const R gcr = (M > 1 && c) ? pc.gr[M - 1] : pc.gr[0];
-        const R gci = (M > 1 && c) ? pc.gi[M - 1] : pc.gi[0];
-        gr += gcr * a2j[0][m];
-        gi += gci * a2j[0][m];
+        gr += pc.gdr * a2j[0][m];
+        gi += pc.gdi * a2j[0][m];
       }
       arg[m] = delta * gr;
       gim[m] = gi;
@@ -285,7 +315,7 @@ __device__ __forceinline__ bool phase_only(const PointTerms& pt, const PointCach
 template <typename R, int M>
 __device__ __forceinline__ void phase_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1, const PointCache<R, M>& pc) {
   const R twod = (R)2 * (R)h1.delta;  // exact doubling of the half-step's delta
-  const R gcr = (M > 1 && c) ? pc.gr[M - 1] : pc.gr[0];
+  const R gcr = pc.gdr;
   R arg[kE], s[kE], co[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) arg[m] = twod * (pc.cr + gcr * abs2(v[m]));
@@ -335,7 +365,7 @@ template <typename R, int M>
 __device__ __forceinline__ void load_ed(EdVals<R>& ev, const EdDesc& d, int c, int64_t a0, int64_t s0, int64_t a1,
                                         int64_t s1) {
   if (d.form == 0) return;
-  if (M > 1 && d.form >= 3) {
+  if (M == 2 && d.form >= 3) {
     const CxPair<R>* rec = reinterpret_cast<const CxPair<R>*>(d.tab);
 #pragma unroll
     for (int m = 0; m < kE; ++m) {
@@ -362,7 +392,7 @@ __device__ __forceinline__ void apply_ed(Cx<R> (&v)[kE], const EdDesc& d, const 
     for (int m = 0; m < kE; ++m) v[m] = scale(v[m], (R)d.scale);
     return;
   }
-  if (M > 1 && d.form >= 3) {
+  if (M == 2 && d.form >= 3) {
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < kE; ++m) mine[lds_pad(t + m * T)] = v[m];
@@ -420,7 +450,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
   const bool valid = line_raw < a.nlines;
   const int64_t line = valid ? line_raw : a.nlines - 1;
   const int64_t y = line % a.ny;
-  Cx<R>* uc = reinterpret_cast<Cx<R>*>(c ? a.u[M - 1] : a.u[0]) + line * N;
+  Cx<R>* uc = reinterpret_cast<Cx<R>*>(pick<M>(a.u, c)) + line * N;
   Cx<R> v[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
@@ -439,7 +469,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
     // workgroup barrier of its own); no other LDS use follows
     if (M > 1 && a.pt.g_coupled) __syncthreads();
   }
-  void* fr = c ? a.frame[M - 1] : a.frame[0];
+  void* fr = pick<M>(a.frame, c);
   // stochastic runs keep both half-steps: the noise acts on the components
   bool pair = false;
   if constexpr (!NOISE)
@@ -502,7 +532,7 @@ __global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
   const int r = tid % (C * T);
   const int cx = r % C, t = r / C;
   const int64_t nx = a.nx;
-  Cx<R>* uc = reinterpret_cast<Cx<R>*>(c ? a.u[M - 1] : a.u[0]) + b * nx * N + x0 + cx;
+  Cx<R>* uc = reinterpret_cast<Cx<R>*>(pick<M>(a.u, c)) + b * nx * N + x0 + cx;
   Cx<R> v[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) v[m] = uc[(int64_t)(t + m * T) * nx];
@@ -537,7 +567,7 @@ __global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
   // the inverse transform starts in this one (no barrier between them)
   constexpr int LAST = (fft_exchanges(N) - 1) & 1;
   fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
-  if (M > 1 && a.ed.form >= 3) {
+  if (M == 2 && a.ed.form >= 3) {
     Cx<R>* mine = my + (1 - LAST) * boff;
     const Cx<R>* theirs = partner + (1 - LAST) * boff;
 #pragma unroll
@@ -589,7 +619,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
   const int64_t line_raw = (int64_t)blockIdx.x * W + l;
   const bool valid = line_raw < a.nlines;
   const int64_t line = valid ? line_raw : a.nlines - 1;
-  Cx<R>* uc = reinterpret_cast<Cx<R>*>(c ? a.u[M - 1] : a.u[0]) + line * N;
+  Cx<R>* uc = reinterpret_cast<Cx<R>*>(pick<M>(a.u, c)) + line * N;
   Cx<R> v[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
@@ -637,7 +667,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
   if (valid) {
 #pragma unroll
     for (int m = 0; m < kE; ++m) uc[t + m * T] = v[m];
-    void* fr = c ? a.frame[M - 1] : a.frame[0];
+    void* fr = pick<M>(a.frame, c);
     if (fr != nullptr) {
       Cx<R>* dst = reinterpret_cast<Cx<R>*>(fr) + line * N;
 #pragma unroll

@@ -56,7 +56,7 @@ struct LaunchDesc {
   const void* func = nullptr;
   dim3 grid, block;
   unsigned lds = 0;
-  alignas(16) unsigned char arg[512];
+  alignas(16) unsigned char arg[1024];
 };
 
 }  // namespace gpe

@@ -49,7 +49,7 @@ enum gpe_noise_kind { GPE_NOISE_NONE = 0, GPE_NOISE_COMPLEX = 1, GPE_NOISE_REAL 
 
 typedef struct gpe_problem_desc {
   int32_t ndim;   /* spatial dimensions: 1, 2 or 3 (length(lengths))          */
-  int32_t ncomp;  /* M components: 1 or 2 (length(u0))                        */
+  int32_t ncomp;  /* M components: 1..4 (length(u0)); M*n[d]/8 <= 1024, LDS   */
   int64_t n[3];   /* grid points per spatial dimension (powers of two, 16..4096) */
   double L[3];    /* box lengths (prob.lengths)                               */
   int64_t batch;  /* product of the trailing (trajectory) dimensions, >= 1    */
@@ -77,7 +77,8 @@ int gpe_destroy(gpe_handle* h);
  * FFT order, column-major over the spatial dims, complex FP64:
  *   GPE_FORM_SCALAR: 1 value per k point
  *   GPE_FORM_DIAG:   M values per k point (component fastest)
- *   GPE_FORM_FULL:   M*M values per k point (column-major matrix)
+ *   GPE_FORM_FULL:   M*M values per k point (column-major matrix); for M > 2
+ *                    the off-diagonal entries must be zero (GPE_ERR_UNSUPPORTED)
  *   GPE_FORM_NONE:   no dispersion (D = AdditiveIdentity), D must be NULL.
  * gpe_solve turns it into exp(-i dt D(k)) on the device (per-k matrix
  * exponential for FULL).  Replaces get_exponential(dispersion, ...)

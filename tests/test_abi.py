@@ -69,10 +69,15 @@ def test_create_rejects_unsupported_shapes(ggp):
     from gpe_hip_amd import _lib
 
     lib = ggp.load_library()
-    for ndim, n, ncomp, code in [(3, (16, 16, 16), 1, -2), (1, (100, 1, 1), 1, -2), (1, (8, 1, 1), 1, -2),
-                                 (1, (8192, 1, 1), 1, -2), (2, (64, 64, 1), 3, -2)]:
+    # (ndim, n, ncomp, dtype): 4 dims; a length that is not a power of two,
+    # too short or too long; 5 components; 3 components of 4096 points
+    # (1536 threads per workgroup); 4 FP64 components of 2048 (LDS)
+    for ndim, n, ncomp, dt in [(4, (16, 16, 16), 1, 1), (1, (100, 1, 1), 1, 1), (1, (8, 1, 1), 1, 1),
+                               (1, (8192, 1, 1), 1, 1), (2, (64, 64, 1), 5, 1), (1, (4096, 1, 1), 3, 1),
+                               (2, (64, 2048, 1), 4, 1)]:
+        code = -2
         d = _lib.ProblemDesc()
-        d.ndim, d.ncomp, d.batch, d.dtype = ndim, ncomp, 1, 1
+        d.ndim, d.ncomp, d.batch, d.dtype = ndim, ncomp, 1, dt
         for i in range(3):
             d.n[i], d.L[i] = n[i], 1.0
         h = ctypes.c_void_p()

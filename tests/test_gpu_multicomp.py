@@ -1,0 +1,121 @@
+"""GPU parity with three and four components (docs/src/multicomponent_systems.md:
+`u0 = (psi_1, psi_2, ...)`; diagonal operators as SVectors).
+
+The M components of a line sit in M wavefront groups of one workgroup, as for
+two.  |u_j|^2 crosses components through LDS for the coupled nonlinearity
+G_c = c_c + sum_j g_cj |u_j|^2.  exp(-i dt D) is diagonal for M > 2: a matrix
+D(k) with non-zero off-diagonal entries is refused.  Bars as in
+test_gpu_parity.py; the oracle's M > 2 path is pinned in test_oracle.py
+(decoupled components equal single runs, per-component norm conservation).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import FP32_TOL, FP64_TOL, check, run_both
+
+pytestmark = pytest.mark.gpu
+
+
+def terms_m(M, L, pump=True, noise=False):
+    """Per-component dispersions, complex cross-coupled G, an SVector
+    potential and a Gaussian pump on the even components."""
+    rng = np.random.default_rng(100 + M)
+    g = rng.uniform(-0.5, 1.0, (M, M)) + 1j * rng.uniform(-0.05, 0.0, (M, M))
+    c0 = rng.uniform(-0.2, 0.2, M) - 1j * rng.uniform(0.0, 0.05, M)
+    a = rng.uniform(0.2, 0.6, M)
+    nd = len(L)
+
+    def dispersion(k, p):
+        kk = sum(x ** 2 for x in k)
+        return [a[c] * kk - 0.02j * c + 0.1 * c for c in range(M)]
+
+    def nonlinearity(u, p):
+        return [c0[c] + sum(g[c, j] * abs(u[j]) ** 2 for j in range(M)) for c in range(M)]
+
+    def potential(r, p):
+        return [0.01 * (c + 1) * (r[0] - L[0] / 2) ** 2 + 0 * r[-1] for c in range(M)]
+
+    def pumpf(r, p, t):
+        f = 0.3 * np.exp(-sum((x - Lx / 2) ** 2 for x, Lx in zip(r, L)) / 2.0)
+        return [f if c % 2 == 0 else 0 * f for c in range(M)]
+
+    out = dict(dispersion=dispersion, nonlinearity=nonlinearity, potential=potential)
+    if pump:
+        out["pump"] = pumpf
+    if noise:
+        out["position_noise_func"] = lambda u, r, p: [0.05 * (c + 1) for c in range(M)]
+        out["noise_prototype"] = True
+    return out
+
+
+def rand_fields(M, shape, seed):
+    rng = np.random.default_rng(seed)
+    return tuple(0.4 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(M))
+
+
+@pytest.mark.parametrize("M", [3, 4])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, FP64_TOL), (np.complex64, FP32_TOL)])
+def test_2d_many_components(ggp, orc, M, dtype, tol):
+    shape, L = (64, 32), (8.0, 5.0)
+    out = run_both(ggp, orc, rand_fields(M, shape, M), L, (0, 0.4), 0.02, 4, dtype=dtype, **terms_m(M, L))
+    err = check(*out, tol)
+    print(f"M={M} 2D {np.dtype(dtype).name}: rel L2 {err:.3e}")
+
+
+@pytest.mark.parametrize("M", [3, 4])
+def test_1d_batched_many_components_with_noise(ggp, orc, M):
+    N, B, L = 128, 3, (12.0,)
+    out = run_both(ggp, orc, rand_fields(M, (N, B), 20 + M), L, (0, 0.6), 0.03, 3, seed=314,
+                   **terms_m(M, L, noise=True))
+    err = check(*out, FP64_TOL)
+    print(f"M={M} 1D batched noise: rel L2 {err:.3e}")
+
+
+def test_3d_three_components(ggp, orc):
+    shape, L = (16, 32, 16), (4.0, 6.0, 5.0)
+    out = run_both(ggp, orc, rand_fields(3, shape, 7), L, (0, 0.3), 0.03, 2, **terms_m(3, L))
+    err = check(*out, FP64_TOL)
+    print(f"M=3 3D: rel L2 {err:.3e}")
+
+
+@pytest.mark.parametrize("M,N,dtype", [(3, 2048, np.complex64), (4, 2048, np.complex64), (4, 1024, np.complex128)])
+def test_longest_lines_for_many_components(ggp, orc, M, N, dtype):
+    """The longest line each precision takes at M = 3, 4 (a workgroup holds
+    M * N/8 threads and two LDS buffers of M lines)."""
+    L = (40.0,)
+    tol = FP64_TOL if dtype == np.complex128 else FP32_TOL
+    out = run_both(ggp, orc, rand_fields(M, (N, 2), 30 + M), L, (0, 0.2), 0.02, 2, dtype=dtype,
+                   **terms_m(M, L, pump=False))
+    err = check(*out, tol)
+    print(f"M={M} N={N} {np.dtype(dtype).name}: rel L2 {err:.3e}")
+
+
+def test_many_components_matrix_dispersion_rules(ggp, orc):
+    """A diagonal SMatrix D(k) is taken as its diagonal; one with coupling
+    between components is refused (exp(-i dt D) stays diagonal for M > 2);
+    lines too long for M components are refused at creation."""
+    M, shape, L = 3, (32, 32), (5.0, 5.0)
+
+    def diag_matrix(k, p):
+        kk = k[0] ** 2 + k[1] ** 2
+        z = 0 * kk
+        return [[kk / 2, z, z], [z, kk / 3 - 0.01j, z], [z, z, kk / 4 + 0.2]]
+
+    out = run_both(ggp, orc, rand_fields(M, shape, 3), L, (0, 0.2), 0.02, 2, dispersion=diag_matrix,
+                   nonlinearity=lambda u, p: [abs(u[0]) ** 2, abs(u[1]) ** 2 + 0.5 * abs(u[2]) ** 2, 0 * u[0]])
+    check(*out, FP64_TOL)
+
+    def coupled(k, p):
+        m = diag_matrix(k, p)
+        m[0][2] = 0.1 + 0 * k[0]
+        return m
+
+    u0 = tuple(np.asfortranarray(x) for x in rand_fields(M, shape, 4))
+    with pytest.raises(ggp.GpeUnsupported):
+        ggp.solve(ggp.GrossPitaevskiiProblem(u0, L, dispersion=coupled), ggp.StrangSplitting(), (0, 0.1),
+                  dt=0.05, nsaves=1)
+    long = tuple(np.zeros((2048, 2), np.complex128, order="F") for _ in range(3))
+    with pytest.raises(ggp.GpeUnsupported):
+        ggp.solve(ggp.GrossPitaevskiiProblem(long, (10.0,)), ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1)

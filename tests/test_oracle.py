@@ -519,3 +519,45 @@ def test_two_component_plane_wave_3d_exact():
         a = scipy.linalg.expm(-1j * t * Dq) @ a0
         for c in range(2):
             np.testing.assert_allclose(res[c][..., f], a[c] * wave, atol=1e-13)
+
+
+# ------------------------------------------------------- M > 2 components
+def test_three_components_decoupled_equal_single_runs():
+    """M = 3 with a diagonal (SVector) D, a potential and G_c depending on
+    |u_c|^2 alone: each component evolves as its own one-component problem,
+    which the scalar pins above fix (docs/src/multicomponent_systems.md: the
+    tuple u0 = (psi_1, psi_2, ...) orders the components)."""
+    N, L = 32, 6.0
+    rng = np.random.default_rng(12)
+    u0 = tuple(0.5 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) for _ in range(3))
+    ds = [lambda k: (k[0] ** 2 + k[1] ** 2) / 2, lambda k: 0.3 * k[0] ** 2 - 0.1j, lambda k: 0.2 * k[1] ** 2 + 0.4]
+    gs = [1.0, -0.5 + 0.05j, 0.3]
+    vs = [0.1, 0.0, -0.2]
+    prob = orc.Problem(u0, (L, L), dispersion=lambda k, p: [d(k) for d in ds],
+                       nonlinearity=lambda u, p: [gs[c] * abs(u[c]) ** 2 for c in range(3)],
+                       potential=lambda r, p: [vs[c] * r[0] + 0 * r[1] for c in range(3)])
+    ts, res, _ = orc.solve(prob, (0, 0.5), 0.05, 2)
+    for c in range(3):
+        single = orc.Problem((u0[c],), (L, L), dispersion=lambda k, p, c=c: ds[c](k),
+                             nonlinearity=lambda u, p, c=c: gs[c] * abs(u[0]) ** 2,
+                             potential=lambda r, p, c=c: vs[c] * r[0] + 0 * r[1])
+        _, r1, _ = orc.solve(single, (0, 0.5), 0.05, 2)
+        assert np.max(abs(res[c] - r1[0])) <= 1e-13 * np.max(abs(r1[0]))
+
+
+def test_four_components_coupled_norm_conserved():
+    """M = 4 with real diagonal D, real cross-coupled G (sum_j g_cj |u_j|^2)
+    and a real potential: every factor of the split step is unitary, so the
+    norm of each component is conserved to rounding."""
+    N, L = 64, 8.0
+    rng = np.random.default_rng(13)
+    u0 = tuple(rng.standard_normal(N) + 1j * rng.standard_normal(N) for _ in range(4))
+    g = rng.uniform(-1, 1, (4, 4))
+    prob = orc.Problem(u0, (L,), dispersion=lambda k, p: [(c + 1) * k[0] ** 2 / 4 for c in range(4)],
+                       nonlinearity=lambda u, p: [sum(g[c, j] * abs(u[j]) ** 2 for j in range(4)) for c in range(4)],
+                       potential=lambda r, p: [np.cos(r[0] * (c + 1)) for c in range(4)])
+    ts, res, _ = orc.solve(prob, (0, 2.0), 0.01, 4)
+    for c in range(4):
+        n0 = np.sum(abs(u0[c]) ** 2)
+        for f in range(len(ts)):
+            assert abs(np.sum(abs(res[c][..., f]) ** 2) - n0) <= 1e-12 * n0
