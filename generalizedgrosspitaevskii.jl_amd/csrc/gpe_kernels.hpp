@@ -34,22 +34,27 @@ struct HalfStep {
   uint64_t call;    // Philox draw counter of this half-step
 };
 
-// Per-point terms shared by every real-space half-step.
-struct PointTerms {
+// Per-point terms shared by every real-space half-step, for up to MM
+// components.  Kernels take PointTerms (kMaxM) as an argument; a kernel that
+// builds the terms in registers (the sweep, one component) uses MM = 1 so the
+// struct stays small enough to live in registers.
+template <int MM>
+struct PointTermsT {
   int gmask;   // bit c: component c has a nonlinearity G_c = c_c + sum_j g[c][j] |u_j|^2
   int g_cplx;  // some coefficient of G has an imaginary part (gain/loss)
   int g_coupled;  // some G_c depends on |u_j|^2 with j != c
-  double g[kMaxM][kMaxM][2];
-  double c[kMaxM][2];
+  double g[MM][MM][2];
+  double c[MM][2];
   int pump_kind;                 // 0 none, 1 table, 2 separable
-  const void* pump_a[kMaxM];     // table: plane of the spatial grid; separable: dim-0 factor
-  const void* pump_b[kMaxM];     // separable: dim-1 factor (2D), else null
+  const void* pump_a[MM];        // table: plane of the spatial grid; separable: dim-0 factor
+  const void* pump_b[MM];        // separable: dim-1 factor (2D), else null
   int pot_kind;                  // 0 none, else exp(-i dt/2 V) planes per component
-  const void* pot[kMaxM];
+  const void* pot[MM];
   int noise_kind;                // 0 none, 1 complex, 2 real
-  double eta[kMaxM];
+  double eta[MM];
   uint32_t seed_lo, seed_hi;
 };
+using PointTerms = PointTermsT<kMaxM>;
 
 template <typename R>
 __device__ __forceinline__ Cx<R> ldc(const void* p, int64_t i) {
@@ -70,8 +75,9 @@ __device__ __forceinline__ Cx<R> cxd(const double (&a)[2]) {
 // Entry c (< M) of a per-component argument array.  c is uniform over a
 // wavefront but not known to the compiler as such; a select chain keeps the
 // kernel arguments out of private memory.
-template <int M, typename T>
-__device__ __forceinline__ T pick(const T (&a)[kMaxM], int c) {
+template <int M, typename T, int K>
+__device__ __forceinline__ T pick(const T (&a)[K], int c) {
+  static_assert(M <= K, "more components than entries");
   T r = a[0];
 #pragma unroll
   for (int j = 1; j < M; ++j) r = (c == j) ? a[j] : r;
@@ -84,8 +90,8 @@ __device__ __forceinline__ T pick(const T (&a)[kMaxM], int c) {
 // P = pbase + x, key = seed; words (x,y) feed x, (z,w) feed x + N/2.
 // pbase is the flat index of the line's first point over 2 (line * N/2 for
 // equal lines; offset/2 for concatenated instances of uneven length).
-template <typename R, int N>
-__device__ __forceinline__ void noise_terms(Cx<R> (&nz)[kE], const HalfStep& hs, const PointTerms& pt, int c, R eta,
+template <typename R, int N, typename PT>
+__device__ __forceinline__ void noise_terms(Cx<R> (&nz)[kE], const HalfStep& hs, const PT& pt, int c, R eta,
                                             int t, int T, int64_t pbase) {
   const R a = (R)sqrt(hs.delta) * eta;
 #pragma unroll
@@ -118,11 +124,20 @@ struct PointCache {
   Cx<R> pump[kE];
   Cx<R> pot[kE];
   bool pumped, has_g;
-  R cr, ci, gr[M], gi[M], gdr, gdi, eta;  // row c of g, and its diagonal g_cc
+  R cr, ci, gr[M], gi[M], eta;  // row c of g
 };
 
+// g_cc of the row a thread holds (a select chain: no indexed register array)
 template <typename R, int M>
-__device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PointTerms& pt, int c, int t, int T,
+__device__ __forceinline__ R diag_of(const R (&row)[M], int c) {
+  R d = row[0];
+#pragma unroll
+  for (int j = 1; j < M; ++j) d = (c == j) ? row[j] : d;
+  return d;
+}
+
+template <typename R, int M, typename PT>
+__device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PT& pt, int c, int t, int T,
                                                  int64_t y, int64_t sp_off) {
   const void* pa = pick<M>(pt.pump_a, c);
   pc.pumped = pt.pump_kind != 0 && pa != nullptr;
@@ -161,16 +176,11 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const Poi
   }
   pc.cr = (R)cr;
   pc.ci = (R)ci;
-  double gdr = gr[0], gdi = gi[0];
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     pc.gr[j] = (R)gr[j];
     pc.gi[j] = (R)gi[j];
-    gdr = (c == j) ? gr[j] : gdr;
-    gdi = (c == j) ? gi[j] : gdi;
   }
-  pc.gdr = (R)gdr;
-  pc.gdi = (R)gdi;
   pc.eta = (R)pick<M>(pt.eta, c);
 }
 
@@ -180,8 +190,8 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const Poi
 // components, |u_j|^2 of the other components comes through the LDS slice
 // `a2` (M rows of N values for this thread's line); every thread of the
 // workgroup takes the same branch, so the barriers are uniform.
-template <typename R, int N, int M, bool NOISE>
-__device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep& hs, const PointTerms& pt,
+template <typename R, int N, int M, bool NOISE, typename PT>
+__device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep& hs, const PT& pt,
                                           const PointCache<R, M>& pc, int t, int T, int64_t pbase, R* a2) {
   const R delta = (R)hs.delta;
   R a2j[M][kE];
@@ -219,8 +229,8 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
             gi += pc.gi[j] * a2j[j][m];
           }
         } else {
-          gr += pc.gdr * a2j[0][m];
-          gi += pc.gdi * a2j[0][m];
+          gr += diag_of<R, M>(pc.gr, c) * a2j[0][m];
+          gi += diag_of<R, M>(pc.gi, c) * a2j[0][m];
         }
         R sn, co;
         fast_sincos(delta * gr, sn, co);
@@ -255,8 +265,8 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
           gi += pc.gi[j] * a2j[j][m];
         }
       } else {
-        gr += pc.gdr * a2j[0][m];
-        gi += pc.gdi * a2j[0][m];
+        gr += diag_of<R, M>(pc.gr, c) * a2j[0][m];
+        gi += diag_of<R, M>(pc.gi, c) * a2j[0][m];
       }
       arg[m] = delta * gr;
       gim[m] = gi;
@@ -306,8 +316,8 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
 // the two half-steps up to rounding.  The test is wave-uniform, and without
 // coupling half_step has no barrier, so the components' wavefronts may take
 // different paths.
-template <typename R, int M>
-__device__ __forceinline__ bool phase_only(const PointTerms& pt, const PointCache<R, M>& pc, const HalfStep& h1,
+template <typename R, int M, typename PT>
+__device__ __forceinline__ bool phase_only(const PT& pt, const PointCache<R, M>& pc, const HalfStep& h1,
                                            const HalfStep& h2) {
   return pc.has_g && !pc.pumped && !pt.pot_kind && !pt.g_cplx && !(M > 1 && pt.g_coupled) && h1.delta == h2.delta;
 }
@@ -315,7 +325,7 @@ __device__ __forceinline__ bool phase_only(const PointTerms& pt, const PointCach
 template <typename R, int M>
 __device__ __forceinline__ void phase_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1, const PointCache<R, M>& pc) {
   const R twod = (R)2 * (R)h1.delta;  // exact doubling of the half-step's delta
-  const R gcr = pc.gdr;
+  const R gcr = diag_of<R, M>(pc.gr, c);
   R arg[kE], s[kE], co[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) arg[m] = twod * (pc.cr + gcr * abs2(v[m]));
@@ -991,7 +1001,7 @@ __global__ void __launch_bounds__(W * (N / kE)) sweep1d_kernel(SweepArgs a) {
   EdVals<R> ev;
 #pragma unroll
   for (int m = 0; m < kE; ++m) ev.own[m] = scale(ldc<R>(a.ed, in.off + t + m * T), invn);
-  PointTerms pt;
+  PointTermsT<1> pt;
   memset(&pt, 0, sizeof pt);
   pt.gmask = (in.g[0] != 0.0 || in.g[1] != 0.0 || in.c[0] != 0.0 || in.c[1] != 0.0) ? 1 : 0;
   pt.g_cplx = a.g_cplx;
