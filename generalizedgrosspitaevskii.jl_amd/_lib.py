@@ -30,6 +30,7 @@ GPE_FORM_FULL = 3
 GPE_PUMP_NONE = 0
 GPE_PUMP_TABLE = 1
 GPE_PUMP_SEPARABLE = 2
+GPE_PUMP_SAMPLES = 3
 
 GPE_NOISE_NONE = 0
 GPE_NOISE_COMPLEX = 1

@@ -55,6 +55,10 @@ struct SteadyBufs {
 };
 
 constexpr int kPlanMaxCand = 6;  // (rows, columns) pairs a measured plan times
+// Device form of a D(k) that couples M > 2 components (exp(-i dt D) row-major per k).
+constexpr int kDFormCoupled = 5;
+// Largest sampled-pump table per component (GPE_PUMP_SAMPLES): 32 GiB of HBM.
+constexpr int64_t kMaxPumpSampleBytes = int64_t(32) << 30;
 
 struct gpe_handle {
   gpe_problem_desc d;
@@ -70,7 +74,7 @@ struct gpe_handle {
   void* tw = nullptr;
   int64_t ntw = 0;
   // dispersion
-  int disp_form = GPE_FORM_NONE;  // device form: 1 scalar, 2 diag, 3 full, 4 symmetric 2x2
+  int disp_form = GPE_FORM_NONE;  // device form: 1 scalar, 2 diag, 3 full, 4 symmetric 2x2, 5 coupled M x M
   int ed_fold = 0;                // bit d: D is even in k_d and the table is folded
   int64_t ed_e0 = 0, ed_e1 = 0;   // compact table extents
   double* dD = nullptr;
@@ -94,6 +98,7 @@ struct gpe_handle {
   double gm_c[2][2][2] = {};
   // pump
   int pump_kind = GPE_PUMP_NONE;
+  int64_t pump_samples = 0;  // GPE_PUMP_SAMPLES: sample planes per component
   void* pump_a[kMaxM] = {nullptr, nullptr};  // active profile per component (nullptr: F_c = 0)
   void* pump_b[kMaxM] = {nullptr, nullptr};
   void* pump_buf_a[kMaxM] = {nullptr, nullptr};  // owned storage behind pump_a / pump_b
@@ -362,11 +367,12 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
   if (!D) return fail(GPE_ERR_INVALID, "gpe_set_dispersion: null table");
   const int M = h->M;
   std::vector<double> diag_only;
+  bool coupled = false;  // M > 2 with off-diagonal entries: device form 5
   if (M > 2 && form == GPE_FORM_FULL) {
-    // the k-space pass of M > 2 components applies a diagonal exp(-i dt D):
-    // a matrix D(k) is accepted when its off-diagonal entries are zero
+    // a matrix D(k) whose off-diagonal entries are all zero is applied as its
+    // diagonal (no exchange between components in the k-space pass)
     diag_only.resize((size_t)2 * M * h->nsp);
-    for (int64_t k = 0; k < h->nsp; ++k)
+    for (int64_t k = 0; k < h->nsp && !coupled; ++k)
       for (int a = 0; a < M; ++a)
         for (int b = 0; b < M; ++b) {
           const double* e = D + 2 * ((size_t)k * M * M + a + (size_t)M * b);
@@ -374,18 +380,20 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
             diag_only[2 * ((size_t)k * M + a)] = e[0];
             diag_only[2 * ((size_t)k * M + a) + 1] = e[1];
           } else if (e[0] != 0.0 || e[1] != 0.0) {
-            return fail(GPE_ERR_UNSUPPORTED,
-                        "gpe_set_dispersion: a non-diagonal D(k) needs M <= 2 components (M = %d)", M);
+            coupled = true;
           }
         }
-    D = diag_only.data();
-    form = GPE_FORM_DIAG;
+    if (!coupled) {
+      D = diag_only.data();
+      form = GPE_FORM_DIAG;
+    }
   }
   const int per_in = per_point(form, M);
   const int64_t n0 = h->n0, n1 = h->n1;
   // device form and values per entry
   int dform = form;
   if (M == 1) dform = GPE_FORM_SCALAR;
+  if (coupled) dform = kDFormCoupled;
   if (dform == GPE_FORM_FULL) {
     bool sym = true;
     for (int64_t k = 0; k < h->nsp && sym; ++k) {
@@ -410,7 +418,11 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
   if (even0) fold |= 1;
   if (even1) fold |= 2;
   const int64_t e0 = even0 ? n0 / 2 + 1 : n0, e1 = even1 ? n1 / 2 + 1 : n1;
-  const int per_out = dform == GPE_FORM_SCALAR ? 1 : (dform == GPE_FORM_DIAG ? M : (dform == GPE_FORM_FULL ? 4 : 3));
+  const int per_out = dform == GPE_FORM_SCALAR  ? 1
+                      : dform == GPE_FORM_DIAG  ? M
+                      : dform == GPE_FORM_FULL  ? 4
+                      : dform == kDFormCoupled ? M * M
+                                                : 3;
   std::vector<double> compact((size_t)e0 * e1 * per_out * 2);
   for (int64_t k1 = 0; k1 < e1; ++k1)
     for (int64_t k0 = 0; k0 < e0; ++k0) {
@@ -437,8 +449,10 @@ extern "C" int gpe_set_dispersion(gpe_handle* h, int32_t form, const double* D) 
   int rc = upload_into(&p, &h->dD_cap, compact.data(), compact.size() * sizeof(double));
   h->dD = (double*)p;
   if (rc) return rc;
-  // device records: four values per k for the two-component forms (see load_ed)
-  rc = ensure_buf(&h->ed, &h->ed_cap, (size_t)e0 * e1 * (dform >= 3 ? 4 : per_out) * h->csz);
+  // device records: four values per k for the two-component forms (see load_ed),
+  // M x M (row-major) for the coupled form
+  const int per_dev = dform == kDFormCoupled ? M * M : (dform >= 3 ? 4 : per_out);
+  rc = ensure_buf(&h->ed, &h->ed_cap, (size_t)e0 * e1 * per_dev * h->csz);
   if (rc) return rc;
   h->disp_form = dform;
   h->ed_fold = fold;
@@ -530,13 +544,25 @@ extern "C" int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, 
   }
   h->env.clear();
   h->pump_kind = GPE_PUMP_NONE;
+  h->pump_samples = 0;
   if (kind == GPE_PUMP_NONE) return GPE_OK;
-  if (kind != GPE_PUMP_TABLE && kind != GPE_PUMP_SEPARABLE) return fail(GPE_ERR_INVALID, "gpe_set_pump: kind=%d", kind);
+  if (kind != GPE_PUMP_TABLE && kind != GPE_PUMP_SEPARABLE && kind != GPE_PUMP_SAMPLES)
+    return fail(GPE_ERR_INVALID, "gpe_set_pump: kind=%d", kind);
+  if (kind == GPE_PUMP_SAMPLES) {
+    if (envelope) return fail(GPE_ERR_INVALID, "gpe_set_pump: a sampled pump takes no envelope");
+    if (n_env < 1) return fail(GPE_ERR_INVALID, "gpe_set_pump: a sampled pump needs n_env >= 1 samples");
+    if ((double)n_env * (double)h->nsp * (double)h->csz > (double)kMaxPumpSampleBytes)
+      return fail(GPE_ERR_UNSUPPORTED, "gpe_set_pump: %lld samples of %lld points exceed the %lld-byte limit per component",
+                  (long long)n_env, (long long)h->nsp, (long long)kMaxPumpSampleBytes);
+  }
   if (kind == GPE_PUMP_SEPARABLE && h->ndim == 3)
     return fail(GPE_ERR_UNSUPPORTED, "gpe_set_pump: 3D grids take the table form");
   if (!profile) return fail(GPE_ERR_INVALID, "gpe_set_pump: null profile");
-  if (n_env < 0 || (n_env > 0 && !envelope)) return fail(GPE_ERR_INVALID, "gpe_set_pump: envelope");
-  const int64_t per = kind == GPE_PUMP_TABLE ? h->nsp : (h->n0 + (h->ndim == 2 ? h->n1 : 0));
+  if (kind != GPE_PUMP_SAMPLES && (n_env < 0 || (n_env > 0 && !envelope)))
+    return fail(GPE_ERR_INVALID, "gpe_set_pump: envelope");
+  const int64_t per = kind == GPE_PUMP_TABLE     ? h->nsp
+                    : kind == GPE_PUMP_SAMPLES ? n_env * h->nsp
+                                               : (h->n0 + (h->ndim == 2 ? h->n1 : 0));
   for (int c = 0; c < h->M; ++c) {
     const double* pc = profile + 2 * per * c;
     bool nonzero = false;
@@ -550,8 +576,8 @@ extern "C" int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, 
       return upload_into(dst, cap, f.data(), (size_t)n * 8);
     };
     int rc;
-    if (kind == GPE_PUMP_TABLE) {
-      rc = put(&h->pump_buf_a[c], &h->pump_cap_a[c], pc, h->nsp);
+    if (kind == GPE_PUMP_TABLE || kind == GPE_PUMP_SAMPLES) {
+      rc = put(&h->pump_buf_a[c], &h->pump_cap_a[c], pc, per);
     } else {
       rc = put(&h->pump_buf_a[c], &h->pump_cap_a[c], pc, h->n0);
       if (!rc && h->ndim == 2) rc = put(&h->pump_buf_b[c], &h->pump_cap_b[c], pc + 2 * h->n0, h->n1);
@@ -560,7 +586,10 @@ extern "C" int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, 
     h->pump_a[c] = h->pump_buf_a[c];
     h->pump_b[c] = (kind == GPE_PUMP_SEPARABLE && h->ndim == 2) ? h->pump_buf_b[c] : nullptr;
   }
-  h->env.assign(envelope, envelope + 2 * n_env);
+  if (kind == GPE_PUMP_SAMPLES)
+    h->pump_samples = n_env;
+  else
+    h->env.assign(envelope, envelope + 2 * n_env);
   h->pump_kind = kind;
   return GPE_OK;
 }
@@ -655,6 +684,7 @@ static PointTerms point_terms(const gpe_handle* h) {
   memcpy(pt.g, h->g, sizeof pt.g);
   memcpy(pt.c, h->c, sizeof pt.c);
   pt.pump_kind = h->pump_kind;
+  pt.pump_stride = h->nsp;
   for (int c = 0; c < kMaxM; ++c) {
     pt.pump_a[c] = h->pump_a[c];
     pt.pump_b[c] = h->pump_b[c];
@@ -674,6 +704,7 @@ static void env_at(const gpe_handle* h, int64_t i, double out[2]) {
     out[0] = 1.0;
     out[1] = 0.0;
   } else {
+    i = std::min<int64_t>(i, (int64_t)h->env.size() / 2 - 1);
     out[0] = h->env[2 * i];
     out[1] = h->env[2 * i + 1];
   }
@@ -692,6 +723,11 @@ static HalfStep make_half(const gpe_handle* h, double dt, int64_t s, int half, u
   hs.fnext[0] = q * ex[0];
   hs.fnext[1] = q * ex[1];
   hs.call = call0 + 2 * (uint64_t)(s - 1) + (uint64_t)half;
+  // sample indices stay inside the sampled table also for the plan
+  // measurement's steps 1 and 2 of a one-step solve
+  const int64_t jmax = h->pump_samples > 0 ? h->pump_samples - 1 : 0;
+  hs.jnow = std::min<int64_t>(2 * s - 2 + half, jmax);
+  hs.jnext = std::min<int64_t>(2 * s - 1 + half, jmax);
   return hs;
 }
 
@@ -1049,7 +1085,8 @@ struct GraphExecGuard {
 constexpr int64_t kGraphMinSaves = 8;
 
 static bool graph_eligible(const gpe_handle* h, int64_t sps) {
-  return h->ndim == 2 && h->noise_kind == GPE_NOISE_NONE && h->env.empty() && sps >= 2 && !h->gmat;
+  return h->ndim == 2 && h->noise_kind == GPE_NOISE_NONE && h->env.empty() && h->pump_kind != GPE_PUMP_SAMPLES &&
+         sps >= 2 && !h->gmat;
 }
 
 static bool use_step_graph(const gpe_handle* h, int64_t sps, int64_t nsaves) {
@@ -1074,6 +1111,9 @@ extern "C" int gpe_solve(gpe_handle* h, const void* const* u0, double t0, double
   if (h->pump_kind != GPE_PUMP_NONE && !h->env.empty() && (int64_t)h->env.size() != 2 * (1 + 2 * S))
     return fail(GPE_ERR_STATE, "gpe_solve: pump envelope has %lld samples, this solve needs %lld (gpe_pump_times)",
                 (long long)(h->env.size() / 2), (long long)(1 + 2 * S));
+  if (h->pump_kind == GPE_PUMP_SAMPLES && h->pump_samples != 1 + 2 * S)
+    return fail(GPE_ERR_STATE, "gpe_solve: the sampled pump has %lld samples, this solve needs %lld (gpe_pump_times)",
+                (long long)h->pump_samples, (long long)(1 + 2 * S));
   if (h->ndim == 1 && !h->env.empty()) {
     const int64_t need = (int64_t)h->env.size();
     if (need > h->d_env_cap) {
@@ -1482,6 +1522,8 @@ static int advance_impl(gpe_handle* h, int64_t nsteps, bool per_kernel, double* 
   if (!h->configured) return fail(GPE_ERR_STATE, "gpe_advance: no previous gpe_solve");
   if (nsteps < 1) return fail(GPE_ERR_INVALID, "gpe_advance: nsteps=%lld", (long long)nsteps);
   if (h->gmat) return fail(GPE_ERR_UNSUPPORTED, "gpe_advance: matrix nonlinearity not supported");
+  if (h->pump_kind == GPE_PUMP_SAMPLES)
+    return fail(GPE_ERR_UNSUPPORTED, "gpe_advance: a sampled pump covers the times of one solve only");
   HIPCHK(hipSetDevice(h->d.device));
   const std::vector<double> saved_env = h->env;
   if (!h->env.empty()) {
@@ -1945,7 +1987,8 @@ extern "C" int gpe_steady_state(gpe_handle* h, const void* const* u0, double dt,
   if (!(dt > 0) || !(tol > 0) || check_every < 2 || max_steps < check_every)
     return fail(GPE_ERR_INVALID, "gpe_steady_state: dt=%g tol=%g check_every=%lld max_steps=%lld", dt, tol,
                 (long long)check_every, (long long)max_steps);
-  if (!h->env.empty()) return fail(GPE_ERR_STATE, "gpe_steady_state: the pump envelope must be static");
+  if (!h->env.empty() || h->pump_kind == GPE_PUMP_SAMPLES)
+    return fail(GPE_ERR_STATE, "gpe_steady_state: the pump envelope must be static");
   if (h->gmat) return fail(GPE_ERR_UNSUPPORTED, "gpe_steady_state: matrix nonlinearity not supported");
   HIPCHK(hipSetDevice(h->d.device));
   int rc = prepare_tables(h, dt);

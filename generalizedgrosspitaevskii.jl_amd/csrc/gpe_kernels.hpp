@@ -17,7 +17,8 @@
 // it writes.  The real-space half-step is muladd_kernel! of
 // src/kernels.jl:37-54 for the descriptor forms
 //     G_c(u) = c_c + sum_j g_cj |u_j|^2,  eta_c constant,
-//     F_c(r, tau) = profile_c(r) * envelope(tau).
+//     F_c(r, tau) = profile_c(r) * envelope(tau), or F_c sampled on the grid
+//     at every pump time when it does not factor.
 #pragma once
 #include "gpe_device.hpp"
 #include "gpe_math.hpp"
@@ -32,6 +33,7 @@ struct HalfStep {
   double fnow[2];   // (delta/2) * envelope(previous pump time)   (complex)
   double fnext[2];  // (delta/2) * envelope(this pump time)       (complex)
   uint64_t call;    // Philox draw counter of this half-step
+  int64_t jnow, jnext;  // pump-time sample indices of F_now / F_next (sampled pump)
 };
 
 // Per-point terms shared by every real-space half-step, for up to MM
@@ -45,9 +47,11 @@ struct PointTermsT {
   int g_coupled;  // some G_c depends on |u_j|^2 with j != c
   double g[MM][MM][2];
   double c[MM][2];
-  int pump_kind;                 // 0 none, 1 table, 2 separable
-  const void* pump_a[MM];        // table: plane of the spatial grid; separable: dim-0 factor
+  int pump_kind;                 // 0 none, 1 table, 2 separable, 3 sampled at every pump time
+  const void* pump_a[MM];        // table: plane of the spatial grid; separable: dim-0 factor;
+                                 // sampled: plane j of the spatial grid at pump_stride * j
   const void* pump_b[MM];        // separable: dim-1 factor (2D), else null
+  int64_t pump_stride;           // sampled: points per sample plane
   int pot_kind;                  // 0 none, else exp(-i dt/2 V) planes per component
   const void* pot[MM];
   int noise_kind;                // 0 none, 1 complex, 2 real
@@ -123,6 +127,7 @@ template <typename R, int M>
 struct PointCache {
   Cx<R> pump[kE];
   Cx<R> pot[kE];
+  const Cx<R>* ptab;  // sampled pump: this line's first point in sample plane 0
   bool pumped, has_g;
   R cr, ci, gr[M], gi[M], eta;  // row c of g
 };
@@ -141,7 +146,8 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PT&
                                                  int64_t y, int64_t sp_off) {
   const void* pa = pick<M>(pt.pump_a, c);
   pc.pumped = pt.pump_kind != 0 && pa != nullptr;
-  if (pc.pumped) {
+  pc.ptab = reinterpret_cast<const Cx<R>*>(pa) + sp_off;
+  if (pc.pumped && pt.pump_kind != 3) {
     const void* pbp = pick<M>(pt.pump_b, c);
     const Cx<R> pb = (pt.pump_kind == 2 && pbp) ? ldc<R>(pbp, y) : mk<R>((R)1, (R)0);
 #pragma unroll
@@ -182,6 +188,14 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PT&
     pc.gi[j] = (R)gi[j];
   }
   pc.eta = (R)pick<M>(pt.eta, c);
+}
+
+// F_c at pump-time sample j for the thread's value m (sampled pump); the
+// profile-times-envelope forms keep the profile in pc.pump and the envelope in
+// the half-step's scalars, which here are (delta/2) exactly.
+template <typename R, int M, typename PT>
+__device__ __forceinline__ Cx<R> pump_at(const PT& pt, const PointCache<R, M>& pc, int64_t j, int m, int t, int T) {
+  return (pt.pump_kind == 3) ? pc.ptab[j * pt.pump_stride + t + m * T] : pc.pump[m];
 }
 
 // Real-space half-step (muladd_kernel!, src/kernels.jl:37-54) on the E values
@@ -239,9 +253,9 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
       }
       if (pt.pot_kind) e = e * pc.pot[m];
       Cx<R> w = v[m];
-      if (pc.pumped) w = w + pc.pump[m] * fnow;
+      if (pc.pumped) w = w + pump_at<R, M>(pt, pc, hs.jnow, m, t, T) * fnow;
       w = e * w;
-      if (pc.pumped) w = w + pc.pump[m] * fnext;
+      if (pc.pumped) w = w + pump_at<R, M>(pt, pc, hs.jnext, m, t, T) * fnext;
       v[m] = w + nz[m];
     }
     return;
@@ -293,7 +307,14 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
     for (int m = 0; m < kE; ++m) e[m] = pc.pot[m];
   }
   // e (u + dt/2 F_now) + dt/2 F_next + noise   (src/kernels.jl:47-48)
-  if (pc.pumped) {
+  if (pc.pumped && pt.pump_kind == 3) {
+    const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
+#pragma unroll
+    for (int m = 0; m < kE; ++m) {
+      const Cx<R> w = v[m] + pc.ptab[hs.jnow * pt.pump_stride + t + m * T] * fnow;
+      v[m] = (has_e ? e[m] * w : w) + pc.ptab[hs.jnext * pt.pump_stride + t + m * T] * fnext;
+    }
+  } else if (pc.pumped) {
     const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
     if (has_e) {
 #pragma unroll
@@ -352,11 +373,14 @@ struct EdDesc {
 };
 
 // Row c of the matrix at the thread's k points: own[m] = E_cc (or the scalar
-// or diagonal entry), other[m] = E_c,1-c for the 2x2 forms.
+// or diagonal entry), other[m] = E_c,1-c for the 2x2 forms.  The coupled
+// M x M form (M > 2) keeps the k points instead, (a0 + s0 m, a1 + s1 m), and
+// reads row c where it is applied.
 template <typename R>
 struct EdVals {
   Cx<R> own[kE];
   Cx<R> other[kE];
+  int64_t a0, s0, a1, s1;
 };
 
 __device__ __forceinline__ int64_t ed_index(const EdDesc& d, int64_t k0, int64_t k1) {
@@ -375,6 +399,13 @@ template <typename R, int M>
 __device__ __forceinline__ void load_ed(EdVals<R>& ev, const EdDesc& d, int c, int64_t a0, int64_t s0, int64_t a1,
                                         int64_t s1) {
   if (d.form == 0) return;
+  if (M > 2 && d.form == 5) {
+    ev.a0 = a0;
+    ev.s0 = s0;
+    ev.a1 = a1;
+    ev.s1 = s1;
+    return;
+  }
   if (M == 2 && d.form >= 3) {
     const CxPair<R>* rec = reinterpret_cast<const CxPair<R>*>(d.tab);
 #pragma unroll
@@ -396,10 +427,30 @@ __device__ __forceinline__ void load_ed(EdVals<R>& ev, const EdDesc& d, int c, i
 // exchange is bracketed by workgroup barriers (uniform: the form is).
 template <typename R, int M>
 __device__ __forceinline__ void apply_ed(Cx<R> (&v)[kE], const EdDesc& d, const EdVals<R>& ev, int t, int T,
-                                         Cx<R>* mine, const Cx<R>* theirs) {
+                                         Cx<R>* mine, const Cx<R>* theirs, int c = 0, int cstride = 0) {
   if (d.form == 0) {
 #pragma unroll
     for (int m = 0; m < kE; ++m) v[m] = scale(v[m], (R)d.scale);
+    return;
+  }
+  if (M > 2 && d.form == 5) {
+    // coupled M x M: every component's line goes to LDS (component j of this
+    // line at mine + (j - c) cstride), then row c of E(k) multiplies the M
+    // values at each k; the table holds E row-major per k
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kE; ++m) mine[lds_pad(t + m * T)] = v[m];
+    __syncthreads();
+    const Cx<R>* base = mine - (int64_t)c * cstride;
+#pragma unroll
+    for (int m = 0; m < kE; ++m) {
+      const int64_t row = (ed_index(d, ev.a0 + ev.s0 * m, ev.a1 + ev.s1 * m) * M + c) * M;
+      Cx<R> acc = mk<R>((R)0, (R)0);
+#pragma unroll
+      for (int j = 0; j < M; ++j) acc = acc + ldc<R>(d.tab, row + j) * base[(int64_t)j * cstride + lds_pad(t + m * T)];
+      v[m] = acc;
+    }
+    __syncthreads();
     return;
   }
   if (M == 2 && d.form >= 3) {
@@ -592,7 +643,7 @@ __global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
 #pragma unroll
     for (int m = 0; m < kE; ++m) v[m] = ev.own[m] * v[m] + ev.other[m] * p[m];
   } else {
-    apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
+    apply_ed<R, M>(v, a.ed, ev, t, T, my, partner, c, C * LS);
     line_sync<LOCAL>();
   }
   fft_lines<R, N, T, 1, true, LOCAL, LAST>(as_bundle(v), my, 0, boff, t, tr);
@@ -661,9 +712,11 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
     h.fnext[0] = q * en[1][0];
     h.fnext[1] = q * en[1][1];
     h.call = a.call0 + 2 * (uint64_t)s;
+    h.jnow = e0;
+    h.jnext = e0 + 1;
     half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line * (N / 2), a2);
     fft_lines<R, N, T, 1, false, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
-    apply_ed<R, M>(v, a.ed, ev, t, T, my, partner);
+    apply_ed<R, M>(v, a.ed, ev, t, T, my, partner, c, W * LS);
     line_sync<LOCAL>();
     fft_lines<R, N, T, 1, true, LOCAL>(as_bundle(v), my, 0, boff, t, tr);
     h.fnow[0] = q * en[1][0];
@@ -671,6 +724,8 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
     h.fnext[0] = q * en[2][0];
     h.fnext[1] = q * en[2][1];
     h.call += 1;
+    h.jnow = e0 + 1;
+    h.jnext = e0 + 2;
     half_step<R, N, M, NOISE>(v, c, h, a.pt, pc, t, T, line * (N / 2), a2);
     line_sync<LOCAL>();
   }
@@ -697,7 +752,7 @@ struct EdArgs {
   void* out;        // device table, k1 fastest (ed_index order)
   int64_t npts;
   int64_t e0, e1;   // compact extents: npts = e0 * e1
-  int form;         // 1 scalar, 2 diag, 3 full 2x2, 4 symmetric 2x2 (a, b, d)
+  int form;         // 1 scalar, 2 diag, 3 full 2x2, 4 symmetric 2x2 (a, b, d), 5 coupled M x M (M > 2)
   int M;
   double dt;
   double scale;
@@ -757,6 +812,61 @@ __device__ __forceinline__ void expm2x2_d(const Cx<double> (&A)[4], Cx<double> (
   E[1] = et * (shs * c);
 }
 
+// exp(A) of an M x M complex matrix (M <= kMaxM, row-major with row stride
+// kMaxM) in FP64: scaling and squaring (Higham, SIAM J. Matrix Anal. Appl.
+// 26 (2005)) with a degree-18 Taylor polynomial of A / 2^s, s chosen so that
+// the max-row-sum norm of A / 2^s is <= 1/2 (truncation < 0.5^19 / 19! ~ 2e-23
+// relative to the norm bound).
+__device__ __noinline__ void expm_small_d(const Cx<double> (&A)[kMaxM * kMaxM], Cx<double> (&E)[kMaxM * kMaxM],
+                                          int M) {
+  double nrm = 0.0;
+  for (int r = 0; r < M; ++r) {
+    double row = 0.0;
+    for (int c = 0; c < M; ++c) row += hypot(A[r * kMaxM + c].re, A[r * kMaxM + c].im);
+    nrm = fmax(nrm, row);
+  }
+  int s = 0;
+  while (nrm > 0.5 && s < 60) {
+    nrm *= 0.5;
+    ++s;
+  }
+  const double f = ldexp(1.0, -s);
+  Cx<double> X[kMaxM * kMaxM], T[kMaxM * kMaxM], P[kMaxM * kMaxM];
+  for (int i = 0; i < kMaxM * kMaxM; ++i) {
+    X[i] = scale(A[i], f);
+    E[i] = mk<double>(0.0, 0.0);
+    T[i] = mk<double>(0.0, 0.0);
+  }
+  for (int r = 0; r < M; ++r) {
+    E[r * kMaxM + r] = mk<double>(1.0, 0.0);
+    T[r * kMaxM + r] = mk<double>(1.0, 0.0);
+  }
+  // E = sum_{n <= 18} X^n / n!, term T_n = T_{n-1} X / n
+  for (int n = 1; n <= 18; ++n) {
+    for (int r = 0; r < M; ++r)
+      for (int c = 0; c < M; ++c) {
+        Cx<double> acc = mk<double>(0.0, 0.0);
+        for (int j = 0; j < M; ++j) acc = acc + T[r * kMaxM + j] * X[j * kMaxM + c];
+        P[r * kMaxM + c] = scale(acc, 1.0 / n);
+      }
+    for (int r = 0; r < M; ++r)
+      for (int c = 0; c < M; ++c) {
+        T[r * kMaxM + c] = P[r * kMaxM + c];
+        E[r * kMaxM + c] = E[r * kMaxM + c] + P[r * kMaxM + c];
+      }
+  }
+  for (int q = 0; q < s; ++q) {
+    for (int r = 0; r < M; ++r)
+      for (int c = 0; c < M; ++c) {
+        Cx<double> acc = mk<double>(0.0, 0.0);
+        for (int j = 0; j < M; ++j) acc = acc + E[r * kMaxM + j] * E[j * kMaxM + c];
+        P[r * kMaxM + c] = acc;
+      }
+    for (int r = 0; r < M; ++r)
+      for (int c = 0; c < M; ++c) E[r * kMaxM + c] = P[r * kMaxM + c];
+  }
+}
+
 template <typename R>
 __global__ void ed_table_kernel(EdArgs a) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -772,6 +882,21 @@ __global__ void ed_table_kernel(EdArgs a) {
       const Cx<double> e = cexp_d(mk<double>(a.dt * z.im, -a.dt * z.re));
       out[o * per + i] = mk<R>((R)(e.re * a.scale), (R)(e.im * a.scale));
     }
+  } else if (a.form == 5) {
+    // coupled M x M (M = 3, 4), D column-major per k; E stored row-major
+    Cx<double> A[kMaxM * kMaxM], E[kMaxM * kMaxM];
+    const int M = a.M;
+    for (int col = 0; col < M; ++col)
+      for (int row = 0; row < M; ++row) {
+        const Cx<double> z = D[k * M * M + row + M * col];
+        A[row * kMaxM + col] = mk<double>(a.dt * z.im, -a.dt * z.re);
+      }
+    expm_small_d(A, E, M);
+    for (int row = 0; row < M; ++row)
+      for (int col = 0; col < M; ++col) {
+        const Cx<double> e = E[row * kMaxM + col];
+        out[(o * M + row) * M + col] = mk<R>((R)(e.re * a.scale), (R)(e.im * a.scale));
+      }
   } else if (a.form == 4) {
     // symmetric 2x2 stored as (a, b, d); exp of a symmetric matrix is symmetric
     Cx<double> A[4], E[4];
@@ -817,7 +942,7 @@ struct GmatArgs {
   HalfStep h_end, h_start;
   double g[2][2][2][2]; // g[a][b][j] complex
   double c[2][2][2];    // c[a][b] complex
-  int pump_kind;        // 0 none, 1 table, 2 separable
+  int pump_kind;        // 0 none, 1 table, 2 separable, 3 sampled (plane j at nsp * j)
   const void* pump_a[kMaxM];
   const void* pump_b[kMaxM];
   int noise_kind;       // 0 none, 1 complex, 2 real (as PointTerms)
@@ -849,12 +974,14 @@ __device__ __forceinline__ Cx<double> gmat_noise(const GmatArgs& a, const HalfSt
 }
 
 template <typename R>
-__device__ __forceinline__ Cx<double> gmat_pump(const GmatArgs& a, int comp, int64_t p) {
+__device__ __forceinline__ Cx<double> gmat_pump(const GmatArgs& a, int comp, int64_t p, int64_t j = 0) {
   const void* pa = a.pump_a[comp];
   if (a.pump_kind == 0 || pa == nullptr) return mk<double>(0.0, 0.0);
   const int64_t sp = p % a.nsp;
   Cx<R> v;
-  if (a.pump_kind == 1) {
+  if (a.pump_kind == 3) {
+    v = ldc<R>(pa, j * a.nsp + sp);
+  } else if (a.pump_kind == 1) {
     v = ldc<R>(pa, sp);
   } else {
     const Cx<R> pb = a.pump_b[comp] ? ldc<R>(a.pump_b[comp], sp / a.n0) : mk<R>((R)1, (R)0);
@@ -886,7 +1013,11 @@ __global__ void __launch_bounds__(256) gmat_half_kernel(GmatArgs a) {
     Cx<double> E[4];
     expm2x2_d(A, E);
     const Cx<double> fn = mk<double>(h.fnow[0], h.fnow[1]), fx = mk<double>(h.fnext[0], h.fnext[1]);
-    const Cx<double> w0 = v[0] + F[0] * fn, w1 = v[1] + F[1] * fn;
+    // F at the previous and at this pump time (one profile unless sampled)
+    const bool smp = a.pump_kind == 3;
+    const Cx<double> Fn[2] = {smp ? gmat_pump<R>(a, 0, p, h.jnow) : F[0], smp ? gmat_pump<R>(a, 1, p, h.jnow) : F[1]};
+    const Cx<double> Fx[2] = {smp ? gmat_pump<R>(a, 0, p, h.jnext) : F[0], smp ? gmat_pump<R>(a, 1, p, h.jnext) : F[1]};
+    const Cx<double> w0 = v[0] + Fn[0] * fn, w1 = v[1] + Fn[1] * fn;
     // _mul(cis(-delta G), exp(-i delta V)) of src/kernels.jl:9-15,44: a
     // scalar V scales the matrix; an SVector V makes it the vector E pv, which
     // then multiplies the fields elementwise (SVector * SVector)
@@ -895,16 +1026,16 @@ __global__ void __launch_bounds__(256) gmat_half_kernel(GmatArgs a) {
       const Cx<R> q0 = ldc<R>(a.pot[0], sp), q1 = ldc<R>(a.pot[1], sp);
       const Cx<double> p0 = mk<double>((double)q0.re, (double)q0.im), p1 = mk<double>((double)q1.re, (double)q1.im);
       const Cx<double> e0 = E[0] * p0 + E[2] * p1, e1 = E[1] * p0 + E[3] * p1;
-      v[0] = e0 * w0 + F[0] * fx;
-      v[1] = e1 * w1 + F[1] * fx;
+      v[0] = e0 * w0 + Fx[0] * fx;
+      v[1] = e1 * w1 + Fx[1] * fx;
     } else {
       if (a.pot_kind == 1) {
         const Cx<R> q = ldc<R>(a.pot[0], sp);
         const Cx<double> sv = mk<double>((double)q.re, (double)q.im);
         for (int i = 0; i < 4; ++i) E[i] = E[i] * sv;
       }
-      v[0] = E[0] * w0 + E[2] * w1 + F[0] * fx;
-      v[1] = E[1] * w0 + E[3] * w1 + F[1] * fx;
+      v[0] = E[0] * w0 + E[2] * w1 + Fx[0] * fx;
+      v[1] = E[1] * w0 + E[3] * w1 + Fx[1] * fx;
     }
     if (a.noise_kind) {
       v[0] = v[0] + gmat_noise<R>(a, h, 0, p);

@@ -323,8 +323,35 @@ def noise_eta(f, M, param, spatial_axes):
     return eta.real.copy()
 
 
-def pump_descriptor(prob, r_axes, spatial, times):
-    """Reduce pump(r, param, t) to profile(r) * envelope(t).
+# Host memory a sampled pump may take (FP64 planes of every component).
+MAX_PUMP_SAMPLE_BYTES = 16 << 30
+
+
+def pump_samples(prob, r_axes, spatial, times, table=None):
+    """F(r, t) on the grid at every pump time, for a pump that does not factor
+    into profile(r) * envelope(t): what evaluate_pump! (src/misc.jl:34-42)
+    computes before each half-step, done once per solve.
+
+    Returns (GPE_PUMP_SAMPLES, float64 view of M blocks of len(times) planes
+    (each column-major over the grid), None)."""
+    M = prob.M
+    nsp = int(np.prod(spatial))
+    nbytes = M * len(times) * nsp * 16
+    if nbytes > MAX_PUMP_SAMPLE_BYTES:
+        raise NotImplementedError(
+            f"pump: F(r,t) does not factor and its {len(times)} samples of {nsp} points need {nbytes} bytes "
+            f"(limit {MAX_PUMP_SAMPLE_BYTES}); not supported by libgpe_hip")
+    out = np.empty((M, len(times), nsp), dtype=np.complex128)
+    for j, t in enumerate(times):
+        F = table(t)
+        for c in range(M):
+            out[c, j] = np.asfortranarray(F[c]).ravel(order="F")
+    return L.GPE_PUMP_SAMPLES, np.ascontiguousarray(out).reshape(-1).view(np.float64), None
+
+
+def pump_descriptor(prob, r_axes, spatial, times, allow_samples=True):
+    """Reduce pump(r, param, t) to profile(r) * envelope(t), or, when it does
+    not factor (and allow_samples), to its samples at every pump time.
 
     Returns (kind, profile float64 array, envelope float64 array or None)."""
     M = prob.M
@@ -386,7 +413,9 @@ def pump_descriptor(prob, r_axes, spatial, times):
         Fi = tabs[i] if i in tabs else table(times[i])
         err = np.linalg.norm(Fi - env[i] * P)
         if err > 1e-12 * max(np.linalg.norm(Fi), normP * abs(env[i]), 1e-300):
-            raise NotImplementedError("pump: F(r,t) is not profile(r)*envelope(t); not supported by libgpe_hip")
+            if not allow_samples:
+                raise NotImplementedError("pump: F(r,t) is not profile(r)*envelope(t); not supported here")
+            return pump_samples(prob, r_axes, spatial, times, table)
     static = bool(np.all(env == env[0]))
     if static:
         P = P * env[0]
@@ -631,8 +660,10 @@ class StrangSplittingIterator:
             kind, prof, env = pump_descriptor(prob, rax, spatial, times)
             if kind != L.GPE_PUMP_NONE:
                 self._keep += [prof, env]
+                # samples: one plane per pump time; else envelope samples (0: static)
+                n_env = len(times) if kind == L.GPE_PUMP_SAMPLES else (0 if env is None else len(env) // 2)
                 L.check(lib.gpe_set_pump(h, kind, prof.ctypes.data, None if env is None else env.ctypes.data,
-                                         0 if env is None else len(env) // 2))
+                                         n_env))
         if "noise" in which and prob.noise_prototype is not None:
             proto = prob.noise_prototype
             first = proto[0] if isinstance(proto, (tuple, list)) else proto
@@ -965,7 +996,8 @@ class SweepIterator:
                 if p.pump is None:
                     envs.append(None)
                     continue
-                kind, pr, env = pump_descriptor(p, (direct_grid(p)[0],), (self.n[i],), times)
+                kind, pr, env = pump_descriptor(p, (direct_grid(p)[0],), (self.n[i],), times,
+                                                allow_samples=False)
                 if kind == L.GPE_PUMP_NONE:
                     envs.append(None)
                     continue

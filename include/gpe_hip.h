@@ -43,7 +43,7 @@ enum gpe_dtype { GPE_C64 = 0 /* ComplexF32 fields */, GPE_C128 = 1 /* ComplexF64
  * closures: a scalar, an SVector{M} (elementwise) or an SMatrix{M,M}. */
 enum gpe_form { GPE_FORM_NONE = 0, GPE_FORM_SCALAR = 1, GPE_FORM_DIAG = 2, GPE_FORM_FULL = 3 };
 
-enum gpe_pump_kind { GPE_PUMP_NONE = 0, GPE_PUMP_TABLE = 1, GPE_PUMP_SEPARABLE = 2 };
+enum gpe_pump_kind { GPE_PUMP_NONE = 0, GPE_PUMP_TABLE = 1, GPE_PUMP_SEPARABLE = 2, GPE_PUMP_SAMPLES = 3 };
 
 enum gpe_noise_kind { GPE_NOISE_NONE = 0, GPE_NOISE_COMPLEX = 1, GPE_NOISE_REAL = 2 };
 
@@ -114,7 +114,17 @@ int gpe_set_nonlinearity_matrix(gpe_handle* h, const double* g, const double* c)
  *   GPE_PUMP_SEPARABLE (1D and 2D): profile = M blocks of (n[0] + ... + n[ndim-1])
  *                       complex values; F_c(r) = prod_d p_{c,d}[i_d].
  * envelope: n_env complex FP64 samples at the times gpe_pump_times() lists
- * for the next gpe_solve; n_env == 0 means envelope == 1 (static pump). */
+ * for the next gpe_solve; n_env == 0 means envelope == 1 (static pump).
+ *   GPE_PUMP_SAMPLES:   F(r,t) that does not factor, sampled on the grid at
+ *                       every pump time (what evaluate_pump! of
+ *                       src/misc.jl:34-42 computes before each half-step):
+ *                       profile = M blocks of n_env planes of prod(n) complex
+ *                       values, plane j at the j-th time gpe_pump_times()
+ *                       lists; envelope must be NULL.  The planes live in
+ *                       device memory (at most 32 GiB per component) and cover
+ *                       one gpe_solve: gpe_advance and gpe_steady_state
+ *                       reject them, and the 2D step loop is not replayed
+ *                       as a graph. */
 int gpe_set_pump(gpe_handle* h, int32_t kind, const double* profile, const double* envelope,
                  int64_t n_env);
 

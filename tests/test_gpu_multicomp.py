@@ -3,10 +3,12 @@
 
 The M components of a line sit in M wavefront groups of one workgroup, as for
 two.  |u_j|^2 crosses components through LDS for the coupled nonlinearity
-G_c = c_c + sum_j g_cj |u_j|^2.  exp(-i dt D) is diagonal for M > 2: a matrix
-D(k) with non-zero off-diagonal entries is refused.  Bars as in
-test_gpu_parity.py; the oracle's M > 2 path is pinned in test_oracle.py
-(decoupled components equal single runs, per-component norm conservation).
+G_c = c_c + sum_j g_cj |u_j|^2.  A diagonal exp(-i dt D) needs no exchange; a
+D(k) that couples components (an M x M SMatrix) is exponentiated per k in FP64
+and all M components of a k point meet in LDS.  Bars as in test_gpu_parity.py;
+the oracle's M > 2 path is pinned in test_oracle.py (decoupled components
+equal single runs, per-component norm conservation, closed-form coupled free
+propagation).
 """
 import math
 
@@ -92,10 +94,88 @@ def test_longest_lines_for_many_components(ggp, orc, M, N, dtype):
     print(f"M={M} N={N} {np.dtype(dtype).name}: rel L2 {err:.3e}")
 
 
+def coupled_dispersion(M, a=None, w=0.3):
+    """D(k) = diag(a_c |k|^2 + losses) + w C with a complex non-Hermitian
+    coupling C (no symmetry in k or between components)."""
+    rng = np.random.default_rng(200 + M)
+    a = rng.uniform(0.2, 0.6, M) if a is None else a
+    C = rng.uniform(-1, 1, (M, M)) + 0.3j * rng.uniform(-1, 1, (M, M))
+
+    def disp(k, p):
+        kk = sum(x ** 2 for x in k)
+        z = 0 * kk
+        return [[(a[r] * kk - 0.02j * r if r == c else z) + w * C[r, c] + 0.05 * (r - c) * k[0] for c in range(M)]
+                for r in range(M)]
+
+    return disp
+
+
+@pytest.mark.parametrize("M", [3, 4])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, FP64_TOL), (np.complex64, FP32_TOL)])
+def test_2d_coupled_dispersion(ggp, orc, M, dtype, tol):
+    shape, L = (64, 32), (8.0, 5.0)
+    terms = terms_m(M, L)
+    terms["dispersion"] = coupled_dispersion(M)
+    out = run_both(ggp, orc, rand_fields(M, shape, 50 + M), L, (0, 0.4), 0.02, 4, dtype=dtype, **terms)
+    err = check(*out, tol)
+    print(f"M={M} coupled D 2D {np.dtype(dtype).name}: rel L2 {err:.3e}")
+
+
+@pytest.mark.parametrize("M", [3, 4])
+def test_1d_batched_coupled_dispersion_with_noise(ggp, orc, M):
+    N, B, L = 256, 3, (12.0,)
+    terms = terms_m(M, L, noise=True)
+    terms["dispersion"] = coupled_dispersion(M)
+    out = run_both(ggp, orc, rand_fields(M, (N, B), 60 + M), L, (0, 0.6), 0.03, 3, seed=271, **terms)
+    err = check(*out, FP64_TOL)
+    print(f"M={M} coupled D 1D batched noise: rel L2 {err:.3e}")
+
+
+def test_3d_coupled_dispersion(ggp, orc):
+    shape, L = (16, 32, 16), (4.0, 6.0, 5.0)
+    terms = terms_m(3, L)
+    terms["dispersion"] = coupled_dispersion(3)
+    out = run_both(ggp, orc, rand_fields(3, shape, 70), L, (0, 0.3), 0.03, 2, **terms)
+    err = check(*out, FP64_TOL)
+    print(f"M=3 coupled D 3D: rel L2 {err:.3e}")
+
+
+def test_coupled_dispersion_free_propagation_exact(ggp):
+    """D(k) = |k|^2/2 I + W P, P a non-symmetric real 4 x 4, nothing else: the
+    GPU frames equal the closed form exp(-i t D(k)) u~(k, 0) through P's
+    eigenvectors (FP64, 1e-12); P != P^T, so a transposed row fails."""
+    M, N, L, W = 4, 64, 6.0, 0.7
+    rng = np.random.default_rng(9)
+    P = rng.uniform(-1, 1, (M, M))
+    lam, V = np.linalg.eig(P)
+    Vi = np.linalg.inv(V)
+    u0 = tuple(np.asfortranarray(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) for _ in range(M))
+
+    def disp(k, p):
+        kk = (k[0] ** 2 + k[1] ** 2) / 2
+        z = 0 * kk
+        return [[kk * (a == b) + W * P[a, b] + z for b in range(M)] for a in range(M)]
+
+    ts, res = ggp.solve(ggp.GrossPitaevskiiProblem(u0, (L, L), dispersion=disp), ggp.StrangSplitting(), (0.0, 1.0),
+                        dt=0.05, nsaves=4)
+    k = 2 * np.pi * np.fft.fftfreq(N, L / N)
+    kk = (k[:, None] ** 2 + k[None, :] ** 2) / 2
+    U0 = np.stack([np.fft.fft2(x) for x in u0])
+    worst = 0.0
+    for f in range(1, 5):
+        t = float(ts[f])
+        mix = V @ np.diag(np.exp(-1j * W * lam * t)) @ Vi
+        want = np.exp(-1j * kk * t)[None] * np.einsum("ab,bxy->axy", mix, U0)
+        for c in range(M):
+            got = np.fft.fft2(np.asarray(res[c])[..., f])
+            worst = max(worst, float(np.linalg.norm(got - want[c]) / np.linalg.norm(want[c])))
+    assert worst <= 1e-12, worst
+    print(f"coupled D M=4 free propagation: rel L2 {worst:.3e}")
+
+
 def test_many_components_matrix_dispersion_rules(ggp, orc):
-    """A diagonal SMatrix D(k) is taken as its diagonal; one with coupling
-    between components is refused (exp(-i dt D) stays diagonal for M > 2);
-    lines too long for M components are refused at creation."""
+    """A diagonal SMatrix D(k) is taken as its diagonal (no exchange between
+    components); lines too long for M components are refused at creation."""
     M, shape, L = 3, (32, 32), (5.0, 5.0)
 
     def diag_matrix(k, p):
@@ -107,15 +187,6 @@ def test_many_components_matrix_dispersion_rules(ggp, orc):
                    nonlinearity=lambda u, p: [abs(u[0]) ** 2, abs(u[1]) ** 2 + 0.5 * abs(u[2]) ** 2, 0 * u[0]])
     check(*out, FP64_TOL)
 
-    def coupled(k, p):
-        m = diag_matrix(k, p)
-        m[0][2] = 0.1 + 0 * k[0]
-        return m
-
-    u0 = tuple(np.asfortranarray(x) for x in rand_fields(M, shape, 4))
-    with pytest.raises(ggp.GpeUnsupported):
-        ggp.solve(ggp.GrossPitaevskiiProblem(u0, L, dispersion=coupled), ggp.StrangSplitting(), (0, 0.1),
-                  dt=0.05, nsaves=1)
     long = tuple(np.zeros((2048, 2), np.complex128, order="F") for _ in range(3))
     with pytest.raises(ggp.GpeUnsupported):
         ggp.solve(ggp.GrossPitaevskiiProblem(long, (10.0,)), ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1)

@@ -96,10 +96,40 @@ def test_pump_descriptor_separable_and_static(ggp):
     e = env.view(np.complex128)
     # profile taken at the probe time of largest norm (t = 2): F = P(r) e(t), e(2) = 1
     assert e.shape == (41,) and np.allclose(e, np.sqrt(times / 2.0))
-    # not separable in time
+    # not separable in time: sampled on the grid at every pump time, planes
+    # in time order (sweeps keep the factored forms only)
     p4 = ggp.GrossPitaevskiiProblem((np.zeros(N),), (L,), pump=lambda r, q, t: np.exp(-(r[0] - t) ** 2))
+    ax = P._bcast_axes(ggp.direct_grid(p4))
+    kind, prof, env = P.pump_descriptor(p4, ax, (N,), times)
+    assert kind == Lb.GPE_PUMP_SAMPLES and env is None
+    x = ggp.direct_grid(p4)[0]
+    want = np.exp(-(x[None, :] - times[:, None]) ** 2)
+    assert np.array_equal(prof.view(np.complex128).reshape(len(times), N), want.astype(complex))
     with pytest.raises(NotImplementedError):
-        P.pump_descriptor(p4, P._bcast_axes(ggp.direct_grid(p4)), (N,), times)
+        P.pump_descriptor(p4, ax, (N,), times, allow_samples=False)
+
+
+def test_sampled_pump_planes_are_column_major_per_component(ggp):
+    """Two components, 2D: block c holds plane j of component c, x fastest."""
+    from gpe_hip_amd import _lib as Lb
+    from gpe_hip_amd import problem as P
+
+    N0, N1, L = 8, 4, (4.0, 2.0)
+
+    def pump(r, q, t):
+        f = (r[0] + 10 * r[1]) * (1 + t) + 1j * t * r[0] ** 2
+        return [f, 0 * f + 3 * t]
+
+    p = ggp.GrossPitaevskiiProblem((np.zeros((N0, N1)),) * 2, L, pump=pump)
+    times = np.array([0.0, 0.5, 1.0])
+    kind, prof, env = P.pump_descriptor(p, P._bcast_axes(ggp.direct_grid(p)), (N0, N1), times)
+    assert kind == Lb.GPE_PUMP_SAMPLES
+    blk = prof.view(np.complex128).reshape(2, len(times), N0 * N1)
+    x, y = ggp.direct_grid(p)
+    for j, t in enumerate(times):
+        f0 = (x[:, None] + 10 * y[None, :]) * (1 + t) + 1j * t * x[:, None] ** 2
+        assert np.array_equal(blk[0, j], f0.ravel(order="F"))
+        assert np.array_equal(blk[1, j], np.full(N0 * N1, 3 * t, complex))
 
 
 def test_noise_prototype_without_noise_func_is_an_error(ggp):

@@ -173,6 +173,41 @@ def test_pump_trapezoid_with_reference_times():
     assert np.linalg.norm(naive - frames[-1]) > 1e-3 * np.linalg.norm(frames[-1])
 
 
+def test_pump_trapezoid_non_factoring_pump():
+    """A pump that does not factor (a spot moving across a 2D box with a
+    time-dependent phase), no D, no G: every half-step adds
+    (dt/4)(F(r, tau_prev) + F(r, tau)) at the reference's lagged pump times,
+    pointwise, so the frames are sums of closed-form samples."""
+    N0, N1, L = 16, 8, (4.0, 2.0)
+    x = np.arange(N0) * (L[0] / N0)
+    y = np.arange(N1) * (L[1] / N1)
+
+    def F(X, Y, t):
+        return np.exp(-((X - 1.0 - t) ** 2 + (Y - 1.0) ** 2)) * (1.0 + 0.2j * t)
+
+    pump = lambda r, p, t: F(r[0], r[1], t)
+    dt, nsaves, tspan = 0.1, 3, (0.0, 0.9)
+    ts, res, it = orc.solve(orc.Problem((np.zeros((N0, N1), complex),), L, pump=pump), tspan, dt, nsaves)
+    rdt, sps, _ = orc.resolve_fixed_timestepping(dt, tspan, nsaves)
+    X, Y = x[:, None], y[None, :]
+    tau_prev, t = 0.0, 0.0
+    expect = np.zeros((N0, N1), complex)
+    frames = []
+    for s in range(sps * nsaves):
+        t += rdt
+        for tau in (t + rdt / 2, t + rdt):
+            expect = expect + (rdt / 4) * (F(X, Y, tau_prev) + F(X, Y, tau))
+            tau_prev = tau
+        if (s + 1) % sps == 0:
+            frames.append(expect.copy())
+    for f in range(nsaves):
+        np.testing.assert_allclose(res[0][..., f + 1], frames[f], rtol=1e-13, atol=1e-15)
+    # one profile times an envelope cannot reproduce it
+    P0 = F(X, Y, 0.0)
+    e = np.vdot(P0, frames[-1]) / np.vdot(P0, P0)
+    assert np.linalg.norm(frames[-1] - e * P0) > 1e-2 * np.linalg.norm(frames[-1])
+
+
 def test_norm_conserved_for_hermitian_problem():
     """Real D(k), real G, no pump/noise: every factor is unitary."""
     rng = np.random.default_rng(1)
@@ -561,3 +596,34 @@ def test_four_components_coupled_norm_conserved():
         n0 = np.sum(abs(u0[c]) ** 2)
         for f in range(len(ts)):
             assert abs(np.sum(abs(res[c][..., f]) ** 2) - n0) <= 1e-12 * n0
+
+
+def test_three_components_coupled_dispersion_exact():
+    """D(k) = |k|^2/2 I + W P with a non-symmetric 3 x 3 coupling P and
+    nothing else: every Strang step is exp(-i dt D(k)) on the transforms, so
+    u(t) is closed-form through the eigenvectors of P (independent of the
+    oracle's matrix exponential; P != P^T, so a transposed operand fails)."""
+    N, L, W = 32, 6.0, 0.7
+    P = np.array([[0.0, 1.0, 0.3], [0.2, 0.5, -0.4], [0.9, -0.1, -0.2]])
+    lam, V = np.linalg.eig(P)
+    Vi = np.linalg.inv(V)
+    rng = np.random.default_rng(5)
+    u0 = tuple(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)) for _ in range(3))
+
+    def disp(k, p):
+        kk = (k[0] ** 2 + k[1] ** 2) / 2
+        z = 0 * kk
+        return [[kk * (a == b) + W * P[a, b] + z for b in range(3)] for a in range(3)]
+
+    tspan, dt, nsaves = (0.0, 0.6), 0.05, 3
+    ts, res, _ = orc.solve(orc.Problem(u0, (L, L), dispersion=disp), tspan, dt, nsaves)
+    k = 2 * np.pi * np.fft.fftfreq(N, L / N)
+    kk = (k[:, None] ** 2 + k[None, :] ** 2) / 2
+    U0 = np.stack([np.fft.fft2(x) for x in u0])
+    for f in range(1, nsaves + 1):
+        t = ts[f]
+        mix = V @ np.diag(np.exp(-1j * W * lam * t)) @ Vi
+        want = np.exp(-1j * kk * t)[None] * np.einsum("ab,bxy->axy", mix, U0)
+        for c in range(3):
+            got = np.fft.fft2(res[c][..., f])
+            assert np.linalg.norm(got - want[c]) <= 1e-12 * np.linalg.norm(want[c])
