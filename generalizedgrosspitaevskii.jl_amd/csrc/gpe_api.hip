@@ -828,10 +828,14 @@ static Step1DArgs s1_args(gpe_handle* h, double dt) {
 // the first solve (and again when the terms that select kernel code change),
 // and keeps the fastest.  The shapes change no arithmetic: results are
 // identical bit for bit under every plan.
+// Disjoint bit fields: noise 0, coupled G 1, complex G 2, G mask 3-6 (four
+// components), dispersion form 7-9, potential form 10-11, pump kind 12-13,
+// matrix G 14; bit 15 marks a key as set.
 static uint64_t plan_key_of(const gpe_handle* h) {
-  return 1 + (uint64_t)(h->noise_kind != GPE_NOISE_NONE) + 2 * (uint64_t)h->g_coupled + 4 * (uint64_t)h->g_cplx +
-         8 * (uint64_t)h->gmask + 32 * (uint64_t)h->disp_form + 256 * (uint64_t)h->pot_form +
-         4096 * (uint64_t)h->pump_kind + 65536 * (uint64_t)h->gmat;
+  return (1ull << 15) | (uint64_t)(h->noise_kind != GPE_NOISE_NONE) | ((uint64_t)(h->g_coupled != 0) << 1) |
+         ((uint64_t)(h->g_cplx != 0) << 2) | (((uint64_t)h->gmask & 15u) << 3) |
+         (((uint64_t)h->disp_form & 7u) << 7) | (((uint64_t)h->pot_form & 3u) << 10) |
+         (((uint64_t)h->pump_kind & 3u) << 12) | ((uint64_t)(h->gmat ? 1 : 0) << 14);
 }
 
 // Mean device time of one call of fn (the launches it issues), from HIP

@@ -127,7 +127,7 @@ template <typename R, int M>
 struct PointCache {
   Cx<R> pump[kE];
   Cx<R> pot[kE];
-  const Cx<R>* ptab;  // sampled pump: this line's first point in sample plane 0
+  uint32_t yline;     // the line's index over the spatial grid (sampled pump: plane offset yline * N)
   bool pumped, has_g;
   R cr, ci, gr[M], gi[M], eta;  // row c of g
 };
@@ -146,7 +146,7 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PT&
                                                  int64_t y, int64_t sp_off) {
   const void* pa = pick<M>(pt.pump_a, c);
   pc.pumped = pt.pump_kind != 0 && pa != nullptr;
-  pc.ptab = reinterpret_cast<const Cx<R>*>(pa) + sp_off;
+  pc.yline = (uint32_t)y;
   if (pc.pumped && pt.pump_kind != 3) {
     const void* pbp = pick<M>(pt.pump_b, c);
     const Cx<R> pb = (pt.pump_kind == 2 && pbp) ? ldc<R>(pbp, y) : mk<R>((R)1, (R)0);
@@ -190,12 +190,11 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PT&
   pc.eta = (R)pick<M>(pt.eta, c);
 }
 
-// F_c at pump-time sample j for the thread's value m (sampled pump); the
-// profile-times-envelope forms keep the profile in pc.pump and the envelope in
-// the half-step's scalars, which here are (delta/2) exactly.
-template <typename R, int M, typename PT>
-__device__ __forceinline__ Cx<R> pump_at(const PT& pt, const PointCache<R, M>& pc, int64_t j, int m, int t, int T) {
-  return (pt.pump_kind == 3) ? pc.ptab[j * pt.pump_stride + t + m * T] : pc.pump[m];
+// Sampled pump: the line's first point in sample plane 0 of component c
+// (formed where it is used, so no pointer stays live in the kernels).
+template <typename R, int N, int M, typename PT>
+__device__ __forceinline__ const Cx<R>* sampled_line(const PT& pt, const PointCache<R, M>& pc, int c) {
+  return reinterpret_cast<const Cx<R>*>(pick<M>(pt.pump_a, c)) + (int64_t)pc.yline * N;
 }
 
 // Real-space half-step (muladd_kernel!, src/kernels.jl:37-54) on the E values
@@ -231,6 +230,16 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
     Cx<R> nz[kE];
     noise_terms<R, N>(nz, hs, pt, c, pc.eta, t, T, pbase);
     const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
+    // a sampled pump (F at every pump time) is added around the loop below,
+    // which keeps the profile forms' code and registers unchanged; the
+    // envelope scalars are (delta/2) exactly for it
+    const bool sampled = pc.pumped && pt.pump_kind == 3;
+    const bool profile = pc.pumped && pt.pump_kind != 3;
+    if (sampled) {
+      const Cx<R>* ptab = sampled_line<R, N, M>(pt, pc, c);
+#pragma unroll
+      for (int m = 0; m < kE; ++m) v[m] = v[m] + ptab[hs.jnow * pt.pump_stride + t + m * T] * fnow;
+    }
 #pragma unroll
     for (int m = 0; m < kE; ++m) {
       Cx<R> e = mk<R>((R)1, (R)0);
@@ -253,10 +262,15 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
       }
       if (pt.pot_kind) e = e * pc.pot[m];
       Cx<R> w = v[m];
-      if (pc.pumped) w = w + pump_at<R, M>(pt, pc, hs.jnow, m, t, T) * fnow;
+      if (profile) w = w + pc.pump[m] * fnow;
       w = e * w;
-      if (pc.pumped) w = w + pump_at<R, M>(pt, pc, hs.jnext, m, t, T) * fnext;
-      v[m] = w + nz[m];
+      if (profile) w = w + pc.pump[m] * fnext;
+      v[m] = sampled ? w : w + nz[m];
+    }
+    if (sampled) {
+      const Cx<R>* ptab = sampled_line<R, N, M>(pt, pc, c);
+#pragma unroll
+      for (int m = 0; m < kE; ++m) v[m] = (v[m] + ptab[hs.jnext * pt.pump_stride + t + m * T] * fnext) + nz[m];
     }
     return;
   }
@@ -309,10 +323,11 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
   // e (u + dt/2 F_now) + dt/2 F_next + noise   (src/kernels.jl:47-48)
   if (pc.pumped && pt.pump_kind == 3) {
     const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
+    const Cx<R>* ptab = sampled_line<R, N, M>(pt, pc, c);
 #pragma unroll
     for (int m = 0; m < kE; ++m) {
-      const Cx<R> w = v[m] + pc.ptab[hs.jnow * pt.pump_stride + t + m * T] * fnow;
-      v[m] = (has_e ? e[m] * w : w) + pc.ptab[hs.jnext * pt.pump_stride + t + m * T] * fnext;
+      const Cx<R> w = v[m] + ptab[hs.jnow * pt.pump_stride + t + m * T] * fnow;
+      v[m] = (has_e ? e[m] * w : w) + ptab[hs.jnext * pt.pump_stride + t + m * T] * fnext;
     }
   } else if (pc.pumped) {
     const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
