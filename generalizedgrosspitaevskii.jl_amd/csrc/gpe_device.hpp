@@ -201,6 +201,46 @@ __device__ __forceinline__ void load_twiddles(TwReg<R, N>& tr, int t, const Cx<R
   load_twiddles_pass<R, N, T, 1>(tr, t, tw, tws);
 }
 
+// Twiddles read from the table at each pass instead of held in registers.
+// Kernels of more than 512 threads per workgroup run with at most 128 VGPRs;
+// there the ~40 registers of a long transform's twiddles made the field spill
+// to scratch memory.  Same table entries and index as load_twiddles_pass, so
+// the arithmetic is identical.
+template <typename R, int N>
+struct TwLazy {
+  const Cx<R>* __restrict__ w;
+  int tws;
+};
+
+template <typename R, int N, int T>
+__device__ __forceinline__ void load_twiddles(TwLazy<R, N>& tr, int, const Cx<R>* __restrict__ tw, int tws) {
+  tr.w = tw;
+  tr.tws = tws;
+}
+
+// Twiddle r (1..RADIX-1) of butterfly b in the pass of span NS.
+template <typename R, int N, int T, int NS, int RADIX>
+__device__ __forceinline__ Cx<R> tw_at(const TwReg<R, N>& tr, int, int b, int r) {
+  return tr.w[tw_count_before(N, NS) + b * (RADIX - 1) + r - 1];
+}
+template <typename R, int N, int T, int NS, int RADIX>
+__device__ __forceinline__ Cx<R> tw_at(const TwLazy<R, N>& tr, int t, int b, int r) {
+  const int k = (t + b * T) & (NS - 1);
+  return tr.w[r * (k * (N / (NS * RADIX)) * tr.tws)];
+}
+
+// Twiddle storage of a kernel of `threads` threads per workgroup.
+template <bool LAZY, typename R, int N>
+struct TwPick {
+  using type = TwReg<R, N>;
+};
+template <typename R, int N>
+struct TwPick<true, R, N> {
+  using type = TwLazy<R, N>;
+};
+template <typename R, int N, int THREADS>
+using TwFor = typename TwPick<(THREADS > 512), R, N>::type;
+
 // Orders the LDS writes of a line's threads before their reads.  When the T
 // threads of every line lie inside one wavefront (LOCAL), a wavefront-scope
 // fence around a wave barrier suffices: a wavefront's LDS instructions execute
@@ -222,13 +262,12 @@ __device__ __forceinline__ void line_sync() {
 // line c of the bundle lives at lds + c*lstride (+ lds_pad(p)).  Two LDS
 // buffers (lds, lds + buf_off) alternate between passes so one barrier per
 // pass suffices.
-template <typename R, int N, int T, int M, bool INV, int NS, int BUF, bool LOCAL>
+template <typename R, int N, int T, int M, bool INV, int NS, int BUF, bool LOCAL, typename TW>
 __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int lstride, int buf_off, int t,
-                                              const TwReg<R, N>& tr) {
+                                              const TW& tr) {
   constexpr int RADIX = PassRadix<N, NS>::value;
   constexpr int B = kE / RADIX;
   constexpr bool LAST = (NS * RADIX == N);
-  constexpr int off = tw_count_before(N, NS);
   static_assert(kE % RADIX == 0, "radix must divide the per-thread count");
   Cx<R>* buf = lds + BUF * buf_off;
 #pragma unroll
@@ -243,7 +282,7 @@ __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < RADIX; ++r) {
-          const Cx<R> w = tr.w[off + b * (RADIX - 1) + r - 1];
+          const Cx<R> w = tw_at<R, N, T, NS, RADIX>(tr, t, b, r);
           a[r] = a[r] * (INV ? conjx(w) : w);
         }
       }
@@ -264,19 +303,19 @@ __device__ __forceinline__ void stockham_pass(Cx<R> (&v)[M][kE], Cx<R>* lds, int
     for (int c = 0; c < M; ++c)
 #pragma unroll
       for (int m = 0; m < kE; ++m) v[c][m] = buf[c * lstride + lds_pad(t + m * T)];
-    stockham_pass<R, N, T, M, INV, NS * RADIX, 1 - BUF, LOCAL>(v, lds, lstride, buf_off, t, tr);
+    stockham_pass<R, N, T, M, INV, NS * RADIX, 1 - BUF, LOCAL, TW>(v, lds, lstride, buf_off, t, tr);
   }
 }
 
 // Unnormalised DFT (INV=false: exp(-2 pi i jk/N); INV=true: exp(+2 pi i jk/N))
 // of M bundled lines, input and output in the cyclic register layout.  LOCAL:
 // the caller guarantees each line's threads share one wavefront.
-template <typename R, int N, int T, int M, bool INV, bool LOCAL = false, int BUF0 = 0>
+template <typename R, int N, int T, int M, bool INV, bool LOCAL = false, int BUF0 = 0, typename TW = TwReg<R, N>>
 __device__ __forceinline__ void fft_lines(Cx<R> (&v)[M][kE], Cx<R>* lds, int lstride, int buf_off, int t,
-                                          const TwReg<R, N>& tr) {
+                                          const TW& tr) {
   static_assert(N / T == kE, "E = N/T must be 8");
   static_assert(!LOCAL || (T <= 64 && 64 % T == 0), "a wave-local line must fit in one wavefront");
-  stockham_pass<R, N, T, M, INV, 1, BUF0, LOCAL>(v, lds, lstride, buf_off, t, tr);
+  stockham_pass<R, N, T, M, INV, 1, BUF0, LOCAL, TW>(v, lds, lstride, buf_off, t, tr);
 }
 
 // ------------------------------------------------------------------- Philox

@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
   Cx<R> v[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
-  TwReg<R, N> tr;
+  TwFor<R, N, M * W * (N / kE)> tr;
   if (a.flags & (kInK | kOutK)) load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
   PointCache<R, M> pc;
   if (a.flags & (kEnd | kStart)) load_point_cache<R, M>(pc, a.pt, c, t, T, y, y * N);
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(M * C * (N / kE)) y_pass_kernel(YPassArgs a) {
   Cx<R> v[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) v[m] = uc[(int64_t)(t + m * T) * nx];
-  TwReg<R, N> tr;
+  TwFor<R, N, M * C * (N / kE)> tr;
   load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
   EdVals<R> ev;
   if (a.kx_n) {
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
   Cx<R> v[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
-  TwReg<R, N> tr;
+  TwFor<R, N, M * W * (N / kE)> tr;
   load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
   PointCache<R, M> pc;
   load_point_cache<R, M>(pc, a.pt, c, t, T, 0, 0);
