@@ -78,7 +78,9 @@ int gpe_destroy(gpe_handle* h);
  *   GPE_FORM_SCALAR: 1 value per k point
  *   GPE_FORM_DIAG:   M values per k point (component fastest)
  *   GPE_FORM_FULL:   M*M values per k point (column-major matrix); for M > 2
- *                    the off-diagonal entries must be zero (GPE_ERR_UNSUPPORTED)
+ *                    a matrix whose off-diagonal entries are all zero is applied
+ *                    as its diagonal, any other is exponentiated per k in FP64
+ *                    (scaling and squaring) and couples all M components
  *   GPE_FORM_NONE:   no dispersion (D = AdditiveIdentity), D must be NULL.
  * gpe_solve turns it into exp(-i dt D(k)) on the device (per-k matrix
  * exponential for FULL).  Replaces get_exponential(dispersion, ...)
