@@ -123,10 +123,15 @@ __device__ __forceinline__ void noise_terms(Cx<R> (&nz)[kE], const HalfStep& hs,
 // pump profile and exp(-i dt/2 V) at the thread's points (loaded at kernel
 // start, so their latency overlaps the field loads) and the nonlinearity
 // coefficients of row c in the field precision.
-template <typename R, int M>
-struct PointCache {
-  Cx<R> pump[kE];
-  Cx<R> pot[kE];
+//
+// LAZY (kernels of more than 512 threads per workgroup, which run with at most
+// 128 VGPRs): the pump profile and exp(-i dt/2 V) are read where the half-step
+// uses them instead of held in 16 registers per value pair (same loads and
+// products, so the same values).
+template <typename R, int M, bool LAZY = false>
+struct PointCacheT {
+  Cx<R> pump[LAZY ? 1 : kE];
+  Cx<R> pot[LAZY ? 1 : kE];
   uint32_t yline;     // the line's index over the spatial grid (sampled pump: plane offset yline * N)
   bool pumped, has_g;
   R cr, ci, gr[M], gi[M], eta;  // row c of g
@@ -141,25 +146,53 @@ __device__ __forceinline__ R diag_of(const R (&row)[M], int c) {
   return d;
 }
 
-template <typename R, int M, typename PT>
-__device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PT& pt, int c, int t, int T,
+template <typename R, int M>
+using PointCache = PointCacheT<R, M, false>;
+
+// Pump profile and exp(-i dt/2 V) at value m of a thread of component c (the
+// line's spatial offset is yline * N: y * N in the 2D/3D row pass, 0 in 1D).
+template <typename R, int N, int M, bool LZ, typename PT>
+__device__ __forceinline__ Cx<R> pump_of(const PT& pt, const PointCacheT<R, M, LZ>& pc, int c, int m, int t, int T) {
+  if constexpr (!LZ) {
+    return pc.pump[m];
+  } else {
+    const int x = t + m * T;
+    const void* pa = pick<M>(pt.pump_a, c);
+    if (pt.pump_kind == 1) return ldc<R>(pa, (int64_t)pc.yline * N + x);
+    const void* pbp = pick<M>(pt.pump_b, c);
+    const Cx<R> pb = (pt.pump_kind == 2 && pbp) ? ldc<R>(pbp, pc.yline) : mk<R>((R)1, (R)0);
+    return ldc<R>(pa, x) * pb;
+  }
+}
+template <typename R, int N, int M, bool LZ, typename PT>
+__device__ __forceinline__ Cx<R> pot_of(const PT& pt, const PointCacheT<R, M, LZ>& pc, int c, int m, int t, int T) {
+  if constexpr (!LZ)
+    return pc.pot[m];
+  else
+    return ldc<R>(pick<M>(pt.pot, c), (int64_t)pc.yline * N + t + m * T);
+}
+
+template <typename R, int M, typename PT, bool LZ>
+__device__ __forceinline__ void load_point_cache(PointCacheT<R, M, LZ>& pc, const PT& pt, int c, int t, int T,
                                                  int64_t y, int64_t sp_off) {
   const void* pa = pick<M>(pt.pump_a, c);
   pc.pumped = pt.pump_kind != 0 && pa != nullptr;
   pc.yline = (uint32_t)y;
-  if (pc.pumped && pt.pump_kind != 3) {
-    const void* pbp = pick<M>(pt.pump_b, c);
-    const Cx<R> pb = (pt.pump_kind == 2 && pbp) ? ldc<R>(pbp, y) : mk<R>((R)1, (R)0);
+  if constexpr (!LZ) {
+    if (pc.pumped && pt.pump_kind != 3) {
+      const void* pbp = pick<M>(pt.pump_b, c);
+      const Cx<R> pb = (pt.pump_kind == 2 && pbp) ? ldc<R>(pbp, y) : mk<R>((R)1, (R)0);
 #pragma unroll
-    for (int m = 0; m < kE; ++m) {
-      const int x = t + m * T;
-      pc.pump[m] = (pt.pump_kind == 1) ? ldc<R>(pa, sp_off + x) : ldc<R>(pa, x) * pb;
+      for (int m = 0; m < kE; ++m) {
+        const int x = t + m * T;
+        pc.pump[m] = (pt.pump_kind == 1) ? ldc<R>(pa, sp_off + x) : ldc<R>(pa, x) * pb;
+      }
     }
-  }
-  if (pt.pot_kind) {
-    const void* pp = pick<M>(pt.pot, c);
+    if (pt.pot_kind) {
+      const void* pp = pick<M>(pt.pot, c);
 #pragma unroll
-    for (int m = 0; m < kE; ++m) pc.pot[m] = ldc<R>(pp, sp_off + t + m * T);
+      for (int m = 0; m < kE; ++m) pc.pot[m] = ldc<R>(pp, sp_off + t + m * T);
+    }
   }
   pc.has_g = (pt.gmask >> c) & 1;
   double cr = pt.c[0][0], ci = pt.c[0][1];
@@ -192,8 +225,8 @@ __device__ __forceinline__ void load_point_cache(PointCache<R, M>& pc, const PT&
 
 // Sampled pump: the line's first point in sample plane 0 of component c
 // (formed where it is used, so no pointer stays live in the kernels).
-template <typename R, int N, int M, typename PT>
-__device__ __forceinline__ const Cx<R>* sampled_line(const PT& pt, const PointCache<R, M>& pc, int c) {
+template <typename R, int N, int M, typename PT, bool LZ>
+__device__ __forceinline__ const Cx<R>* sampled_line(const PT& pt, const PointCacheT<R, M, LZ>& pc, int c) {
   return reinterpret_cast<const Cx<R>*>(pick<M>(pt.pump_a, c)) + (int64_t)pc.yline * N;
 }
 
@@ -203,9 +236,9 @@ __device__ __forceinline__ const Cx<R>* sampled_line(const PT& pt, const PointCa
 // components, |u_j|^2 of the other components comes through the LDS slice
 // `a2` (M rows of N values for this thread's line); every thread of the
 // workgroup takes the same branch, so the barriers are uniform.
-template <typename R, int N, int M, bool NOISE, typename PT>
+template <typename R, int N, int M, bool NOISE, typename PT, bool LZ>
 __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep& hs, const PT& pt,
-                                          const PointCache<R, M>& pc, int t, int T, int64_t pbase, R* a2) {
+                                          const PointCacheT<R, M, LZ>& pc, int t, int T, int64_t pbase, R* a2) {
   const R delta = (R)hs.delta;
   R a2j[M][kE];
   if (pt.gmask) {
@@ -260,11 +293,11 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
         const R mag = pt.g_cplx ? (R)exp(delta * gi) : (R)1;
         e = mk<R>(mag * co, -mag * sn);
       }
-      if (pt.pot_kind) e = e * pc.pot[m];
+      if (pt.pot_kind) e = e * pot_of<R, N, M>(pt, pc, c, m, t, T);
       Cx<R> w = v[m];
-      if (profile) w = w + pc.pump[m] * fnow;
+      if (profile) w = w + pump_of<R, N, M>(pt, pc, c, m, t, T) * fnow;
       w = e * w;
-      if (profile) w = w + pc.pump[m] * fnext;
+      if (profile) w = w + pump_of<R, N, M>(pt, pc, c, m, t, T) * fnext;
       v[m] = sampled ? w : w + nz[m];
     }
     if (sampled) {
@@ -314,11 +347,11 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
     }
     if (pt.pot_kind) {
 #pragma unroll
-      for (int m = 0; m < kE; ++m) e[m] = e[m] * pc.pot[m];
+      for (int m = 0; m < kE; ++m) e[m] = e[m] * pot_of<R, N, M>(pt, pc, c, m, t, T);
     }
   } else if (pt.pot_kind) {
 #pragma unroll
-    for (int m = 0; m < kE; ++m) e[m] = pc.pot[m];
+    for (int m = 0; m < kE; ++m) e[m] = pot_of<R, N, M>(pt, pc, c, m, t, T);
   }
   // e (u + dt/2 F_now) + dt/2 F_next + noise   (src/kernels.jl:47-48)
   if (pc.pumped && pt.pump_kind == 3) {
@@ -333,10 +366,16 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
     const Cx<R> fnow = cxd<R>(hs.fnow), fnext = cxd<R>(hs.fnext);
     if (has_e) {
 #pragma unroll
-      for (int m = 0; m < kE; ++m) v[m] = e[m] * (v[m] + pc.pump[m] * fnow) + pc.pump[m] * fnext;
+      for (int m = 0; m < kE; ++m) {
+        const Cx<R> f = pump_of<R, N, M>(pt, pc, c, m, t, T);
+        v[m] = e[m] * (v[m] + f * fnow) + f * fnext;
+      }
     } else {
 #pragma unroll
-      for (int m = 0; m < kE; ++m) v[m] = (v[m] + pc.pump[m] * fnow) + pc.pump[m] * fnext;
+      for (int m = 0; m < kE; ++m) {
+        const Cx<R> f = pump_of<R, N, M>(pt, pc, c, m, t, T);
+        v[m] = (v[m] + f * fnow) + f * fnext;
+      }
     }
   } else if (has_e) {
 #pragma unroll
@@ -352,14 +391,15 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
 // the two half-steps up to rounding.  The test is wave-uniform, and without
 // coupling half_step has no barrier, so the components' wavefronts may take
 // different paths.
-template <typename R, int M, typename PT>
-__device__ __forceinline__ bool phase_only(const PT& pt, const PointCache<R, M>& pc, const HalfStep& h1,
+template <typename R, int M, typename PT, bool LZ>
+__device__ __forceinline__ bool phase_only(const PT& pt, const PointCacheT<R, M, LZ>& pc, const HalfStep& h1,
                                            const HalfStep& h2) {
   return pc.has_g && !pc.pumped && !pt.pot_kind && !pt.g_cplx && !(M > 1 && pt.g_coupled) && h1.delta == h2.delta;
 }
 
-template <typename R, int M>
-__device__ __forceinline__ void phase_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1, const PointCache<R, M>& pc) {
+template <typename R, int M, bool LZ>
+__device__ __forceinline__ void phase_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1,
+                                           const PointCacheT<R, M, LZ>& pc) {
   const R twod = (R)2 * (R)h1.delta;  // exact doubling of the half-step's delta
   const R gcr = diag_of<R, M>(pc.gr, c);
   R arg[kE], s[kE], co[kE];
@@ -532,7 +572,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) x_pass_kernel(XPassArgs a) {
   for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
   TwFor<R, N, M * W * (N / kE)> tr;
   if (a.flags & (kInK | kOutK)) load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
-  PointCache<R, M> pc;
+  PointCacheT<R, M, (M * W * T > 512)> pc;
   if (a.flags & (kEnd | kStart)) load_point_cache<R, M>(pc, a.pt, c, t, T, y, y * N);
   Cx<R>* my = lds + (c * W + l) * LS;
   const int boff = M * W * LS;
@@ -701,7 +741,7 @@ __global__ void __launch_bounds__(M * W * (N / kE)) step1d_kernel(Step1DArgs a) 
   for (int m = 0; m < kE; ++m) v[m] = uc[t + m * T];
   TwFor<R, N, M * W * (N / kE)> tr;
   load_twiddles<R, N, T>(tr, t, reinterpret_cast<const Cx<R>*>(a.tw), a.tws);
-  PointCache<R, M> pc;
+  PointCacheT<R, M, (M * W * T > 512)> pc;
   load_point_cache<R, M>(pc, a.pt, c, t, T, 0, 0);
   EdVals<R> ev;
   load_ed<R, M>(ev, a.ed, c, t, T, 0, 0);

@@ -190,3 +190,22 @@ def test_many_components_matrix_dispersion_rules(ggp, orc):
     long = tuple(np.zeros((2048, 2), np.complex128, order="F") for _ in range(3))
     with pytest.raises(ggp.GpeUnsupported):
         ggp.solve(ggp.GrossPitaevskiiProblem(long, (10.0,)), ggp.StrangSplitting(), (0, 0.1), dt=0.05, nsaves=1)
+
+
+@pytest.mark.parametrize("M,shape,dtype,noise", [
+    (1, (4096, 16), np.complex64, False),    # 2D rows of 4096 points, 1024-thread row pass
+    (2, (4096, 2), np.complex64, True),      # 1D whole-step kernel, two 4096-point components
+    (3, (2048, 2), np.complex64, False),     # 1D, three 2048-point components
+    (2, (2048, 2), np.complex128, False),    # 1D FP64, two 2048-point components
+])
+def test_long_lines_with_pump_and_potential(ggp, orc, M, shape, dtype, noise):
+    """Kernels of more than 512 threads per workgroup read the pump profile,
+    exp(-i dt/2 V) and the FFT twiddles where they are used; with every term
+    on, their frames match the oracle as the register-held forms do."""
+    nd = 2 if shape[1] == 16 else 1
+    L = (60.0, 12.0) if nd == 2 else (60.0,)
+    tol = FP64_TOL if dtype == np.complex128 else FP32_TOL
+    out = run_both(ggp, orc, rand_fields(M, shape, 80 + M), L, (0, 0.2), 0.02, 2, dtype=dtype,
+                   seed=(123 if noise else None), **terms_m(M, L, pump=True, noise=noise))
+    err = check(*out, tol)
+    print(f"M={M} {shape} {np.dtype(dtype).name} pump+V: rel L2 {err:.3e}")
