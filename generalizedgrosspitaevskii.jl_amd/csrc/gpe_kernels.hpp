@@ -135,15 +135,21 @@ struct PointCacheT {
   uint32_t yline;     // the line's index over the spatial grid (sampled pump: plane offset yline * N)
   bool pumped, has_g;
   R cr, ci, gr[M], gi[M], eta;  // row c of g
+  // g_cc for M > 2, taken from the kernel arguments: selecting it from gr[] at
+  // use let the compiler index that array by c, which put the whole cache in
+  // scratch memory (for M <= 2 the select stays a select and costs no registers)
+  R gdr, gdi;
 };
 
-// g_cc of the row a thread holds (a select chain: no indexed register array)
-template <typename R, int M>
-__device__ __forceinline__ R diag_of(const R (&row)[M], int c) {
-  R d = row[0];
-#pragma unroll
-  for (int j = 1; j < M; ++j) d = (c == j) ? row[j] : d;
-  return d;
+// g_cc (real or imaginary part) of the row a thread holds
+template <typename R, int M, bool IM, bool LZ>
+__device__ __forceinline__ R gdiag(const PointCacheT<R, M, LZ>& pc, int c) {
+  if constexpr (M > 2) {
+    return IM ? pc.gdi : pc.gdr;
+  } else {
+    const R(&row)[M] = IM ? pc.gi : pc.gr;
+    return (M == 2 && c == 1) ? row[M - 1] : row[0];
+  }
 }
 
 template <typename R, int M>
@@ -220,6 +226,16 @@ __device__ __forceinline__ void load_point_cache(PointCacheT<R, M, LZ>& pc, cons
     pc.gr[j] = (R)gr[j];
     pc.gi[j] = (R)gi[j];
   }
+  if constexpr (M > 2) {
+    double dr = pt.g[0][0][0], di = pt.g[0][0][1];
+#pragma unroll
+    for (int i = 1; i < M; ++i) {
+      dr = (c == i) ? pt.g[i][i][0] : dr;
+      di = (c == i) ? pt.g[i][i][1] : di;
+    }
+    pc.gdr = (R)dr;
+    pc.gdi = (R)di;
+  }
   pc.eta = (R)pick<M>(pt.eta, c);
 }
 
@@ -285,8 +301,8 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
             gi += pc.gi[j] * a2j[j][m];
           }
         } else {
-          gr += diag_of<R, M>(pc.gr, c) * a2j[0][m];
-          gi += diag_of<R, M>(pc.gi, c) * a2j[0][m];
+          gr += gdiag<R, M, false>(pc, c) * a2j[0][m];
+          gi += gdiag<R, M, true>(pc, c) * a2j[0][m];
         }
         R sn, co;
         fast_sincos(delta * gr, sn, co);
@@ -326,8 +342,8 @@ __device__ __forceinline__ void half_step(Cx<R> (&v)[kE], int c, const HalfStep&
           gi += pc.gi[j] * a2j[j][m];
         }
       } else {
-        gr += diag_of<R, M>(pc.gr, c) * a2j[0][m];
-        gi += diag_of<R, M>(pc.gi, c) * a2j[0][m];
+        gr += gdiag<R, M, false>(pc, c) * a2j[0][m];
+        gi += gdiag<R, M, true>(pc, c) * a2j[0][m];
       }
       arg[m] = delta * gr;
       gim[m] = gi;
@@ -401,7 +417,7 @@ template <typename R, int M, bool LZ>
 __device__ __forceinline__ void phase_pair(Cx<R> (&v)[kE], int c, const HalfStep& h1,
                                            const PointCacheT<R, M, LZ>& pc) {
   const R twod = (R)2 * (R)h1.delta;  // exact doubling of the half-step's delta
-  const R gcr = diag_of<R, M>(pc.gr, c);
+  const R gcr = gdiag<R, M, false>(pc, c);
   R arg[kE], s[kE], co[kE];
 #pragma unroll
   for (int m = 0; m < kE; ++m) arg[m] = twod * (pc.cr + gcr * abs2(v[m]));
